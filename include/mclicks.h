@@ -1,0 +1,178 @@
+/* mclicks.h -- C ABI of the MI355X EM engine for the PBM, UBM, CCM and DBN
+ * click models.
+ *
+ * Each entry point replaces one piece of the reference's click-model
+ * interface or EM driver; the reference file:line it stands in for is given
+ * beside it.  All pointers are host pointers; sizes are element counts.
+ * Every call returns an mc_status; mc_last_error() returns a thread-local
+ * message describing the most recent failure on the calling thread.  No call
+ * exits the process or throws across the ABI (the reference aborts through
+ * CUDA_CHECK/MPI_Abort, utils/utils.cuh:27-42).
+ *
+ * Threading: one mc_model is bound to one device and one HIP stream; calls on
+ * a model must be serialised by the caller.  Different models may be used
+ * from different threads.
+ */
+#ifndef MCLICKS_H
+#define MCLICKS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MC_MAX_SERP 10 /* utils/definitions.h:50 */
+
+/* Model ids as on the reference command line (main.cpp:178-183). */
+enum { MC_PBM = 0, MC_CCM = 1, MC_DBN = 2, MC_UBM = 3 };
+
+typedef enum {
+    MC_OK = 0,
+    MC_ERR_INVALID_ARG = 1,
+    MC_ERR_OOM = 2,
+    MC_ERR_HIP = 3,
+    MC_ERR_IO = 4,
+    MC_ERR_STATE = 5,
+    MC_ERR_UNSUPPORTED = 6
+} mc_status;
+
+/* Options for mc_model_set_option. */
+enum {
+    /* 0 = deterministic segmented reduction (default), 1 = float atomics. */
+    MC_OPT_REDUCE_MODE = 0,
+    /* Target sessions per workgroup work item (default 2048). */
+    MC_OPT_ITEM_SESSIONS = 1,
+    /* Per-qd accumulator window per work item, in entries (default: auto). */
+    MC_OPT_ITEM_WINDOW = 2,
+    /* 1 = reorder sessions inside each work item by last-click rank so that
+     * wavefronts run uniform chains (default 1). */
+    MC_OPT_SORT_BY_LCR = 3,
+    /* Workgroup size of the E/M kernel: 64, 128, 256 or 512 (default 256). */
+    MC_OPT_BLOCK = 4
+};
+enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };
+
+/* ---------------------------------------------------------------------------
+ * Dataset preparation (host).  Replaces Dataset::parse_dataset
+ * (data/dataset.cpp:421-455), which parses the raw log (dataset.cpp:224-305),
+ * splits train/test (dataset.cpp:385-396), sorts the train set by query
+ * (dataset.cpp:17-43) and assigns query-document indices
+ * (dataset.cpp:314-374).  One worker: the hash ring sends every session to
+ * device 0.
+ * ------------------------------------------------------------------------- */
+typedef struct mc_dataset mc_dataset;
+
+/* max_sessions == -1 reads every complete session (main.cpp -s flag). */
+mc_status mc_dataset_parse(const char* path, int max_sessions, float test_share, mc_dataset** out);
+
+/* Same preparation for sessions already in memory: query[n], docs[n*10],
+ * clicks[n*10] (0/1), in log order. */
+mc_status mc_dataset_from_sessions(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
+                                   float test_share, mc_dataset** out);
+
+mc_status mc_dataset_sizes(const mc_dataset* ds, int64_t* n_parsed, int64_t* n_train, int64_t* n_test,
+                           int64_t* n_qd);
+
+/* which: 0 = train (sorted by query), 1 = test (filtered, log order).
+ * Arrays sized n, n*10, n*10, n*10.  pidx is -1 for unseen test pairs. */
+mc_status mc_dataset_export(const mc_dataset* ds, int which, int32_t* query, int32_t* docs, uint8_t* clicks,
+                            int32_t* pidx);
+
+void mc_dataset_free(mc_dataset* ds);
+
+/* ---------------------------------------------------------------------------
+ * Click model on one device.
+ * ------------------------------------------------------------------------- */
+typedef struct mc_model mc_model;
+
+/* create_cm_host (click_models/base.cu:19-49).  device = HIP ordinal. */
+mc_status mc_model_create(int model_type, int device, mc_model** out);
+
+mc_status mc_model_set_option(mc_model* m, int option, int64_t value);
+
+/* ClickModel_Hst::init_parameters (base.cuh:39, common.cu:31-61) together
+ * with the one-time dataset upload (parallel_em.cu:248-253).
+ * query[n] must group equal queries contiguously (the reference sorts the
+ * train set by query); pidx[n*10] and clicks[n*10] are session-major;
+ * len[n] (1..10) may be NULL for full SERPs.  Tables start at (1,2). */
+mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const int32_t* pidx,
+                             const uint8_t* clicks, const uint8_t* len, int64_t n_qd);
+
+/* Convenience: init_parameters from a prepared dataset's train set. */
+mc_status mc_init_from_dataset(mc_model* m, const mc_dataset* ds);
+
+/* ClickModel_Hst::process_session (base.cuh:36), i.e. Kernel::em_training
+ * (parallel_em/kernel.cu:42-64): the E-step over all training sessions with
+ * the expected counts pre-reduced per work item. */
+mc_status mc_process_session(mc_model* m);
+
+/* reset_parameters + update_parameters (base.cuh:37,43; parallel_em.cu:503-547):
+ * theta <- (1,2) + sum of the E-step's expected counts. */
+mc_status mc_update_parameters(mc_model* m);
+
+/* n_iter x (E-step, M-step) as in parallel_em.cu:461-609.  use_graph != 0
+ * replays the whole run as one captured HIP graph. */
+mc_status mc_em_iterations(mc_model* m, int n_iter, int use_graph);
+
+/* Same as mc_em_iterations(use_graph = 0) but timed with HIP events on the
+ * model's stream: total wall time and the average E-step/M-step kernel time. */
+mc_status mc_em_iterations_timed(mc_model* m, int n_iter, double* ms_total, double* ms_estep_avg,
+                                 double* ms_mstep_avg);
+
+/* Times n_iter iterations replayed from one captured graph (after one warm
+ * replay). */
+mc_status mc_em_graph_timed(mc_model* m, int n_iter, double* ms_total);
+
+mc_status mc_synchronize(mc_model* m);
+
+/* get_parameter_information (base.cuh:33): table t's name, whether it is
+ * public (rank-indexed, shared by all queries) and its length.  Tables are
+ * numbered per-qd first: attractiveness (all), satisfaction (DBN), then the
+ * public one: examination (PBM 10, UBM 55), tau (CCM 3), gamma (DBN 1). */
+mc_status mc_get_n_tables(const mc_model* m, int* n_tables);
+mc_status mc_get_table_info(const mc_model* m, int t, const char** name, int* is_public, int64_t* size);
+
+/* get_parameters + transfer_parameters(D2H) (base.cuh:40-41, common.cu:93-102). */
+mc_status mc_get_parameters(mc_model* m, int t, float* num, float* den);
+
+/* set_parameters + transfer_parameters(H2D) (base.cuh:40,42). */
+mc_status mc_set_parameters(mc_model* m, int t, const float* num, const float* den);
+
+/* reset_parameters (base.cuh:43, common.cu:71-82): every table to (1,2). */
+mc_status mc_reset_parameters(mc_model* m);
+
+/* Held-out evaluation on the device: get_log_conditional_click_probs and
+ * get_full_click_probs per session (e.g. dbn.cu:611-683), LogLikelihood and
+ * Perplexity (click_models/evaluation.cpp:25-66).  pidx[n*10] (-1 = unseen
+ * pair, valued 1/2), clicks[n*10], len[n] (NULL = 10).  Outputs: logp/full
+ * (n*10, nullable), llh[n] per-session mean log-probability (nullable),
+ * *llh_sum = sum of llh in FP64, ppl_log2_sum[10] = per-rank sums of
+ * log2(full) in FP64, rank_count[10] = sessions contributing to each rank.
+ * LL = llh_sum / n; PPL_r = 2^(-ppl_log2_sum[r] / rank_count[r]). */
+mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                      float* logp, float* full, float* llh, double* llh_sum, double* ppl_log2_sum,
+                      int64_t* rank_count);
+
+/* get_memory_usage (base.cuh:31): device bytes held by the model. */
+mc_status mc_get_memory_usage(const mc_model* m, size_t* bytes);
+
+/* compute_memory_footprint (base.cuh:32) for n sessions and n_qd pairs. */
+mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, size_t* bytes);
+
+/* Work decomposition chosen at init: number of work items, number of per-qd
+ * entries merged across items, and the largest item in sessions. */
+mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
+                             int64_t* max_item_sessions);
+
+/* destroy_parameters (base.cuh:44) and the model object. */
+void mc_model_destroy(mc_model* m);
+
+const char* mc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCLICKS_H */

@@ -1,0 +1,331 @@
+// mc_dataset.cpp -- host-side dataset preparation of the EM engine.
+//
+// Turns a raw click log (or in-memory sessions) into the training and test
+// sets the reference builds in Dataset::parse_dataset (data/dataset.cpp:421-455):
+//   * parsing: query lines of 15 tokens open a SERP, click lines of 4 tokens
+//     mark the first rank showing the clicked document, a SERP is emitted only
+//     when the next query line arrives (dataset.cpp:224-305, search.cu:93-122);
+//   * split: n_train = int(n * (1 - test_share) - 0.5) + 1, the tail is the test
+//     set (dataset.cpp:385-396);
+//   * the train set is ordered by std::sort on the query id
+//     (dataset.cpp:17-21); std::sort is not stable, so the same library
+//     algorithm is used here to reproduce the reference's order of equal keys;
+//   * query-document indices in first-occurrence order over the sorted train
+//     set (dataset.cpp:314-339); test sessions with an unseen query or no seen
+//     document are dropped, unseen documents keep index -1 (dataset.cpp:349-374).
+//
+// Two undefined behaviours of the reference parser get a defined meaning
+// here: empty lines are skipped, and lines before the first query line never
+// emit a SERP (the reference would emit an uninitialised one).
+
+#include <algorithm>
+#include <cerrno>
+#include <climits>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mc_internal.h"
+
+namespace mc {
+
+namespace {
+
+constexpr int kQueryTokens = 15;   // utils/definitions.h:51
+constexpr int kClickTokens = 4;    // utils/definitions.h:52
+
+inline bool is_space(char c) {
+    return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
+}
+
+enum IntParse { kIntOk = 0, kIntInvalid = 1, kIntRange = 2 };
+
+// std::stoi on one whitespace-free token: optional sign, decimal digits, stops
+// at the first non-digit; no digits -> invalid_argument; outside int -> out_of_range.
+IntParse parse_int(const char* b, const char* e, int* out) {
+    const char* p = b;
+    bool neg = false;
+    if (p < e && (*p == '+' || *p == '-')) { neg = *p == '-'; p++; }
+    if (p >= e || *p < '0' || *p > '9') return kIntInvalid;
+    long long v = 0;
+    bool overflow = false;
+    for (; p < e && *p >= '0' && *p <= '9'; p++) {
+        if (!overflow) {
+            v = v * 10 + (*p - '0');
+            if (v > (long long)INT_MAX + 1) overflow = true;
+        }
+    }
+    if (neg) v = -v;
+    if (overflow || v > INT_MAX || v < INT_MIN) return kIntRange;
+    *out = (int)v;
+    return kIntOk;
+}
+
+// Open-addressing int -> int map with O(1) clear through generation stamps;
+// used to number the documents of one query group.
+class GroupDocMap {
+public:
+    void reset(size_t expected) {
+        size_t want = 16;
+        while (want < expected * 2) want <<= 1;
+        if (want > keys_.size()) {
+            keys_.assign(want, 0);
+            vals_.assign(want, 0);
+            gen_.assign(want, 0);
+            cur_ = 0;
+        }
+        cur_++;
+        if (cur_ == 0) { std::fill(gen_.begin(), gen_.end(), 0u); cur_ = 1; }
+        size_ = 0;
+    }
+    // Returns the stored index for key, inserting next_index if absent.
+    int find_or_insert(int key, int next_index, bool* inserted) {
+        if ((size_ + 1) * 2 > keys_.size()) grow();
+        size_t mask = keys_.size() - 1;
+        size_t h = hash(key) & mask;
+        while (gen_[h] == cur_) {
+            if (keys_[h] == key) { *inserted = false; return vals_[h]; }
+            h = (h + 1) & mask;
+        }
+        gen_[h] = cur_;
+        keys_[h] = key;
+        vals_[h] = next_index;
+        size_++;
+        *inserted = true;
+        return next_index;
+    }
+
+private:
+    static size_t hash(int k) { return (size_t)((uint32_t)k * 2654435761u) ^ ((uint32_t)k >> 16); }
+    void grow() {
+        std::vector<int> k2, v2;
+        for (size_t i = 0; i < keys_.size(); i++)
+            if (gen_[i] == cur_) { k2.push_back(keys_[i]); v2.push_back(vals_[i]); }
+        size_t n = keys_.size() * 2;
+        keys_.assign(n, 0);
+        vals_.assign(n, 0);
+        gen_.assign(n, 0);
+        cur_ = 1;
+        size_ = 0;
+        bool ins;
+        for (size_t i = 0; i < k2.size(); i++) find_or_insert(k2[i], v2[i], &ins);
+    }
+    std::vector<int> keys_, vals_;
+    std::vector<uint32_t> gen_;
+    uint32_t cur_ = 0;
+    size_t size_ = 0;
+};
+
+// Flat (query, doc) -> index hash used to look up test pairs.
+class PairIndex {
+public:
+    void init(size_t n) {
+        size_t cap = 16;
+        while (cap < n * 2) cap <<= 1;
+        keys_.assign(cap, kEmpty);
+        vals_.assign(cap, -1);
+    }
+    void insert(int q, int d, int idx) {
+        uint64_t key = pack(q, d);
+        size_t mask = keys_.size() - 1, h = hash(key) & mask;
+        while (keys_[h] != kEmpty && keys_[h] != key) h = (h + 1) & mask;
+        keys_[h] = key;
+        vals_[h] = idx;
+    }
+    int find(int q, int d) const {
+        uint64_t key = pack(q, d);
+        size_t mask = keys_.size() - 1, h = hash(key) & mask;
+        while (keys_[h] != kEmpty) {
+            if (keys_[h] == key) return vals_[h];
+            h = (h + 1) & mask;
+        }
+        return -1;
+    }
+
+private:
+    static constexpr uint64_t kEmpty = ~0ull;
+    static uint64_t pack(int q, int d) { return ((uint64_t)(uint32_t)q << 32) | (uint32_t)d; }
+    static size_t hash(uint64_t k) {
+        k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
+        return (size_t)k;
+    }
+    std::vector<uint64_t> keys_;
+    std::vector<int> vals_;
+};
+
+}  // namespace
+
+// Split, sort and index `all` (log order) into ds.  Returns false with an
+// error message when the split is empty.
+bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const std::vector<int32_t>& d_all,
+                     const std::vector<uint8_t>& c_all, float test_share, std::string* err) {
+    const int n = (int)q_all.size();
+    ds.n_parsed = n;
+    if (n < 1) { *err = "no complete session"; return false; }
+    const int n_train = static_cast<int>(n * (1 - test_share) - 0.5) + 1;   // dataset.cpp:390
+    if (n_train < 1 || n_train > n) {
+        *err = "test share leaves an invalid train/test split";
+        return false;
+    }
+    // Order of the train set: libstdc++ std::sort on the query key
+    // (dataset.cpp:19-20).  The permutation depends only on the comparison
+    // outcomes, so sorting (query, position) pairs reproduces it exactly.
+    struct Key { int32_t q; int32_t pos; };
+    std::vector<Key> keys((size_t)n_train);
+    for (int i = 0; i < n_train; i++) keys[i] = Key{q_all[i], i};
+    std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) { return a.q < b.q; });
+
+    ds.train_query.resize((size_t)n_train);
+    ds.train_docs.resize((size_t)n_train * kMaxSerp);
+    ds.train_clicks.resize((size_t)n_train * kMaxSerp);
+    ds.train_pidx.resize((size_t)n_train * kMaxSerp);
+    for (int i = 0; i < n_train; i++) {
+        int src = keys[i].pos;
+        ds.train_query[i] = q_all[src];
+        std::memcpy(&ds.train_docs[(size_t)i * kMaxSerp], &d_all[(size_t)src * kMaxSerp], kMaxSerp * 4);
+        std::memcpy(&ds.train_clicks[(size_t)i * kMaxSerp], &c_all[(size_t)src * kMaxSerp], kMaxSerp);
+    }
+
+    // First-occurrence indices; equal queries are contiguous after the sort,
+    // so each query group is numbered with a group-local map.
+    GroupDocMap gmap;
+    int next = 0;
+    for (int s = 0; s < n_train;) {
+        int e = s;
+        while (e < n_train && ds.train_query[e] == ds.train_query[s]) e++;
+        gmap.reset((size_t)std::min(e - s, 64) * kMaxSerp);
+        for (int i = s; i < e; i++) {
+            for (int r = 0; r < kMaxSerp; r++) {
+                bool ins;
+                int idx = gmap.find_or_insert(ds.train_docs[(size_t)i * kMaxSerp + r], next, &ins);
+                if (ins) next++;
+                ds.train_pidx[(size_t)i * kMaxSerp + r] = idx;
+            }
+        }
+        s = e;
+    }
+    ds.n_qd = next;
+
+    // Test set: keep log order, drop sessions whose query is unseen or whose
+    // documents are all unseen.
+    PairIndex pairs;
+    pairs.init((size_t)ds.n_qd);
+    std::vector<int32_t> train_queries;
+    train_queries.reserve(1024);
+    for (int s = 0; s < n_train;) {
+        int e = s;
+        while (e < n_train && ds.train_query[e] == ds.train_query[s]) e++;
+        train_queries.push_back(ds.train_query[s]);
+        for (int i = s; i < e; i++)
+            for (int r = 0; r < kMaxSerp; r++)
+                pairs.insert(ds.train_query[i], ds.train_docs[(size_t)i * kMaxSerp + r],
+                             ds.train_pidx[(size_t)i * kMaxSerp + r]);
+        s = e;
+    }
+    // train_queries is ascending (sorted train set).
+    ds.test_query.clear();
+    ds.test_docs.clear();
+    ds.test_clicks.clear();
+    ds.test_pidx.clear();
+    for (int i = n_train; i < n; i++) {
+        int q = q_all[i];
+        if (!std::binary_search(train_queries.begin(), train_queries.end(), q)) continue;
+        int32_t pid[kMaxSerp];
+        bool found = false;
+        for (int r = 0; r < kMaxSerp; r++) {
+            pid[r] = pairs.find(q, d_all[(size_t)i * kMaxSerp + r]);
+            if (pid[r] != -1) found = true;
+        }
+        if (!found) continue;
+        ds.test_query.push_back(q);
+        ds.test_docs.insert(ds.test_docs.end(), &d_all[(size_t)i * kMaxSerp], &d_all[(size_t)i * kMaxSerp] + kMaxSerp);
+        ds.test_clicks.insert(ds.test_clicks.end(), &c_all[(size_t)i * kMaxSerp],
+                              &c_all[(size_t)i * kMaxSerp] + kMaxSerp);
+        ds.test_pidx.insert(ds.test_pidx.end(), pid, pid + kMaxSerp);
+    }
+    return true;
+}
+
+bool parse_raw_log(const char* path, int max_sessions, std::vector<int32_t>* q_all, std::vector<int32_t>* d_all,
+                   std::vector<uint8_t>* c_all, std::string* err) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) { *err = std::string("cannot open ") + path; return false; }
+    std::string buf;
+    {
+        char chunk[1 << 16];
+        size_t got;
+        while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.append(chunk, got);
+        std::fclose(f);
+    }
+    const char* p = buf.data();
+    const char* end = p + buf.size();
+
+    int32_t cur_q = -1, cur_d[kMaxSerp];
+    uint8_t cur_c[kMaxSerp];
+    bool have_cur = false;
+    int n_queries = 0, sid_prev = -1;
+    const char* tb[kQueryTokens + 1];
+    const char* te[kQueryTokens + 1];
+    long line_no = 0;
+
+    while (p < end && (n_queries < max_sessions || max_sessions == -1)) {
+        const char* nl = (const char*)std::memchr(p, '\n', (size_t)(end - p));
+        const char* le = nl ? nl : end;
+        line_no++;
+        // Tokenise (std::istringstream >> std::string).
+        int nt = 0;
+        const char* t = p;
+        while (t < le) {
+            while (t < le && is_space(*t)) t++;
+            if (t >= le) break;
+            const char* s = t;
+            while (t < le && !is_space(*t)) t++;
+            if (nt <= kQueryTokens) { tb[nt] = s; te[nt] = t; }
+            nt++;
+        }
+        p = nl ? nl + 1 : end;
+        if (nt == 0) continue;
+        int sid;
+        IntParse ps = parse_int(tb[0], te[0], &sid);
+        if (ps == kIntInvalid) continue;                      // dataset.cpp:258-263
+        if (ps == kIntRange) { *err = "session id out of range at line " + std::to_string(line_no); return false; }
+        if (nt == kQueryTokens) {
+            if (sid_prev != -1 && have_cur) {                   // dataset.cpp:272-275
+                q_all->push_back(cur_q);
+                d_all->insert(d_all->end(), cur_d, cur_d + kMaxSerp);
+                c_all->insert(c_all->end(), cur_c, cur_c + kMaxSerp);
+                n_queries++;
+            }
+            if (parse_int(tb[3], te[3], &cur_q) != kIntOk) {
+                *err = "bad query id at line " + std::to_string(line_no);
+                return false;
+            }
+            for (int r = 0; r < kMaxSerp; r++) {
+                if (parse_int(tb[r + 5], te[r + 5], &cur_d[r]) != kIntOk) {
+                    *err = "bad document id at line " + std::to_string(line_no);
+                    return false;
+                }
+                cur_c[r] = 0;
+            }
+            have_cur = true;
+        } else if (nt == kClickTokens) {
+            int doc;
+            if (parse_int(tb[3], te[3], &doc) != kIntOk) {
+                *err = "bad clicked document id at line " + std::to_string(line_no);
+                return false;
+            }
+            if (have_cur) {
+                for (int r = 0; r < kMaxSerp; r++) {
+                    if (cur_d[r] == doc) { cur_c[r] = 1; break; }
+                }
+            }
+        }
+        sid_prev = sid;                                          // dataset.cpp:288
+    }
+    return true;
+}
+
+}  // namespace mc

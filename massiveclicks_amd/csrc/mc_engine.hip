@@ -1,0 +1,1339 @@
+// mc_engine.hip -- kernels and C ABI of the MI355X EM engine.
+//
+// Per EM iteration two kernels run on the model's stream:
+//   em_item_kernel   E-step of one work item per workgroup, per-item
+//                    reduction of the expected counts, in-place M-step of the
+//                    entries the item owns (parallel_em/kernel.cu:42-89 and
+//                    click_models/common.cu:114-192 fused);
+//   em_merge_kernel  M-step of entries merged across items and of the shared
+//                    (rank-indexed) tables.
+// theta_t is read from buffer `cur`, theta_{t+1} written to `cur ^ 1`, so the
+// (1,2) reset of the reference (common.cu:71-82, a host fill plus H2D per
+// iteration) disappears: the M-step writes prior + sum directly.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mclicks.h"
+#include "mc_internal.h"
+#include "mc_kernels.h"
+
+namespace mc {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+#define MC_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                     \
+            return MC_ERR_HIP;                                                                \
+        }                                                                                     \
+    } while (0)
+
+#define MC_REQUIRE(cond, msg)                                                                 \
+    do {                                                                                      \
+        if (!(cond)) {                                                                        \
+            set_error(msg);                                                                   \
+            return MC_ERR_INVALID_ARG;                                                        \
+        }                                                                                     \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Number of per-qd tables, shared-table length and shared accumulators.
+template <int MODEL> struct Shape;
+template <> struct Shape<kPBM> { static constexpr int T = 1, K = 10, KACC = 10; };
+template <> struct Shape<kUBM> { static constexpr int T = 1, K = 55, KACC = 55; };
+template <> struct Shape<kDBN> { static constexpr int T = 2, K = 1, KACC = 2; };
+template <> struct Shape<kCCM> { static constexpr int T = 1, K = 3, KACC = 6; };
+
+template <int MODEL, bool VARLEN, class Emit>
+__device__ __forceinline__ void run_session(const KArgs& A, int64_t s, Emit& em, const float* sh, float (&acc)[55]) {
+    const uint32_t m = A.mask[s];
+    const int L = VARLEN ? (int)A.len[s] : kL;
+    const uint32_t mm = m & ((1u << L) - 1u);
+    if (MODEL == kPBM) {
+        pbm_session(A, s, L, mm, em, sh, acc);
+    } else if (MODEL == kUBM) {
+        ubm_session(A, s, L, mm, em, sh, acc);
+    } else if (MODEL == kDBN) {
+        if (mm == 0) return;   // every count of a click-free session is a constant
+        float gn = 0.f, gd = 0.f;
+        dbn_session<VARLEN>(A, s, L, mm, em, sh[0], gn, gd);
+        acc[0] += gn;
+        acc[1] += gd;
+    } else {
+        if (mm == 0) return;
+        float ta[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        ccm_session<VARLEN>(A, s, L, mm, em, sh[0], sh[1], sh[2], ta);
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[k] += ta[k];
+    }
+}
+
+// Wave-level sum of a 64-bit integer (wave64).
+__device__ __forceinline__ long long wave_sum(long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int MODEL, bool VARLEN>
+__global__ void __launch_bounds__(512) em_item_kernel(KArgs A) {
+    constexpr int T = Shape<MODEL>::T;
+    constexpr int K = Shape<MODEL>::K;
+    constexpr int KACC = Shape<MODEL>::KACC;
+    extern __shared__ unsigned long long lds[];
+    __shared__ float sh[K < 64 ? 64 : K];
+    __shared__ long long red[KACC][8];
+
+    const Item it = A.items[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
+    const int W = it.width;
+    if (A.mode == 0 && W > 0) {
+        for (int i = tid; i < T * W; i += nthr) lds[i] = 0ull;
+    }
+    __syncthreads();
+
+    float acc[55];
+#pragma unroll
+    for (int k = 0; k < 55; k++) acc[k] = 0.f;
+
+    if (A.mode == 1) {
+        EmitGlobalFloat em{A.tab[0].gaccf, A.tab[1].gaccf};
+        for (int64_t s = it.s0 + tid; s < it.s1; s += nthr) run_session<MODEL, VARLEN>(A, s, em, sh, acc);
+    } else if (W > 0) {
+        EmitLds em{lds, it.base, W};
+        for (int64_t s = it.s0 + tid; s < it.s1; s += nthr) run_session<MODEL, VARLEN>(A, s, em, sh, acc);
+    } else {
+        EmitGlobalFix em{A.tab[0].gacc, A.tab[1].gacc};
+        for (int64_t s = it.s0 + tid; s < it.s1; s += nthr) run_session<MODEL, VARLEN>(A, s, em, sh, acc);
+    }
+
+    // Shared-table partials: fixed point, wave sums, then across waves.
+    const int wave = tid >> 6, lane = tid & 63, nwave = nthr >> 6;
+#pragma unroll
+    for (int k = 0; k < KACC; k++) {
+        long long v = wave_sum(to_fix(acc[k]));
+        if (lane == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    for (int k = tid; k < KACC; k += nthr) {
+        long long v = 0;
+        for (int w = 0; w < nwave; w++) v += red[k][w];
+        A.sh_partial[(int64_t)k * A.n_items + blockIdx.x] = v;
+    }
+
+    // Flush the LDS window.
+    if (A.mode == 0 && W > 0) {
+        const bool interior = (it.flags & kItemInterior) != 0;
+        for (int i = tid; i < W; i += nthr) {
+            const int p = it.base + i;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                const unsigned long long v = lds[t * W + i];
+                if (interior) {
+                    const float2 bd = A.tab[t].base_den[p];
+                    const float num = (float)((double)bd.x + (double)(long long)v * kFixInv);
+                    A.tab[t].val_out[p] = clamp_value(num, bd.y);
+                    A.tab[t].num_out[p] = num;
+                } else if (v != 0ull) {
+                    atomicAdd(&A.tab[t].gacc[p], v);
+                }
+            }
+        }
+    }
+}
+
+// Merge kernel: blocks [0, K) finalise the shared tables, the rest finalise
+// per-qd entries accumulated in global memory (merged entries in segmented
+// mode, every entry in atomic mode).
+template <int MODEL>
+__global__ void __launch_bounds__(256) em_merge_kernel(KArgs A) {
+    constexpr int T = Shape<MODEL>::T;
+    const int tid = threadIdx.x;
+    if ((int)blockIdx.x < A.K) {
+        const int k = blockIdx.x;
+        __shared__ long long red[2][4];
+        long long sn = 0, sd = 0;
+        const int an = A.sh_acc_map[2 * k], ad = A.sh_acc_map[2 * k + 1];
+        for (int i = tid; i < A.n_items; i += blockDim.x) {
+            if (an >= 0) sn += A.sh_partial[(int64_t)an * A.n_items + i];
+            if (ad >= 0) sd += A.sh_partial[(int64_t)ad * A.n_items + i];
+        }
+        sn = wave_sum(sn);
+        sd = wave_sum(sd);
+        if ((tid & 63) == 0) { red[0][tid >> 6] = sn; red[1][tid >> 6] = sd; }
+        __syncthreads();
+        if (tid == 0) {
+            long long tn = 0, td = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) { tn += red[0][w]; td += red[1][w]; }
+            const float num = (float)(A.sh_base[2 * k] + (double)tn * kFixInv);
+            const float den = (float)(A.sh_base[2 * k + 1] + (double)td * kFixInv);
+            A.sh_num_out[k] = num;
+            A.sh_den_out[k] = den;
+            A.sh_val_out[k] = clamp_value(num, den);
+        }
+        return;
+    }
+    const int64_t i = (int64_t)(blockIdx.x - A.K) * blockDim.x + tid;
+    if (A.mode == 1) {
+        if (i >= A.n_qd) return;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const float2 bd = A.tab[t].base_den[i];
+            const float v = A.tab[t].gaccf[i];
+            const float num = (float)((double)bd.x + (double)v);
+            A.tab[t].val_out[i] = clamp_value(num, bd.y);
+            A.tab[t].num_out[i] = num;
+            A.tab[t].gaccf[i] = 0.f;
+        }
+    } else {
+        if (i >= A.n_shared) return;
+        const int p = A.shared_idx[i];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const float2 bd = A.tab[t].base_den[p];
+            const unsigned long long v = A.tab[t].gacc[p];
+            const float num = (float)((double)bd.x + (double)(long long)v * kFixInv);
+            A.tab[t].val_out[p] = clamp_value(num, bd.y);
+            A.tab[t].num_out[p] = num;
+            A.tab[t].gacc[p] = 0ull;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation kernels (pbm.cu:376-439, ubm.cu:373-468, dbn.cu:611-683,
+// ccm.cu:604-671; evaluation.cpp:25-66).  pidx -1 = pair unseen in training,
+// valued PARAM_DEF_NUM / PARAM_DEF_DENOM.
+// ---------------------------------------------------------------------------
+struct EvalArgs {
+    const int32_t* pidx;     // [n][10]
+    const uint8_t* clicks;   // [n][10]
+    const uint8_t* len;      // [n] or null
+    int64_t n;
+    const float* val0;       // attractiveness values
+    const float* val1;       // satisfaction values (DBN)
+    const float* sh;         // shared values
+    float* logp;             // [n][10] or null
+    float* full;             // [n][10] or null
+    float* llh;              // [n] or null
+    double* partial;         // [n_blocks][21]: llh, ppl[10], count[10]
+};
+
+__device__ __forceinline__ float pv_or_default(const float* v, int p) {
+    return p != -1 ? v[p] : 0.5f;   // PARAM_DEF_NUM / PARAM_DEF_DENOM
+}
+
+template <int MODEL>
+__device__ void eval_session(const EvalArgs& E, int64_t i, int L, float (&lp)[kL], float (&fp)[kL]) {
+    const int32_t* p = E.pidx + i * kL;
+    const uint8_t* c = E.clicks + i * kL;
+    if (MODEL == kPBM) {
+        for (int r = 0; r < L; r++) {
+            float atr = pv_or_default(E.val0, p[r]);
+            float ae = atr * E.sh[r];
+            lp[r] = c[r] == 1 ? logf(ae) : logf(1 - ae);
+            fp[r] = c[r] == 1 ? ae : 1 - ae;
+        }
+    } else if (MODEL == kUBM) {
+        int prev[kL];
+        int last = kL - 1;
+        for (int r = 0; r < L; r++) { prev[r] = last; if (c[r] == 1) last = r; }
+        for (int r = 0; r < L; r++) {
+            float atr = pv_or_default(E.val0, p[r]);
+            float ae = atr * E.sh[r * (r + 1) / 2 + prev[r]];
+            lp[r] = c[r] == 1 ? logf(ae) : logf(1 - ae);
+        }
+        float tf[kL];
+        for (int r = 0; r < L; r++) {
+            float cp = 0.f;
+            for (int rp = -1; rp < r; rp++) {
+                float nb = 1.f;
+                int crp = rp == -1 ? 9 : rp;
+                for (int rb = rp + 1; rb < r; rb++) {
+                    float av = pv_or_default(E.val0, p[rb]);
+                    float ev = E.sh[rb * (rb + 1) / 2 + crp];
+                    nb *= 1 - av * ev;
+                }
+                float av = pv_or_default(E.val0, p[r]);
+                float ev = E.sh[r * (r + 1) / 2 + crp];
+                float temp = nb * (av * ev);
+                cp += rp >= 0 ? tf[rp] * temp : temp;
+            }
+            tf[r] = cp;
+            fp[r] = c[r] == 1 ? cp : 1 - cp;
+        }
+    } else if (MODEL == kDBN) {
+        const float g = E.sh[0];
+        float ex = 1.f;
+        for (int r = 0; r < L; r++) {
+            float a = pv_or_default(E.val0, p[r]);
+            float s = pv_or_default(E.val1, p[r]);
+            float cp;
+            if (c[r] == 1) { cp = a * ex; ex = g * (1 - s); }
+            else { cp = 1 - a * ex; ex *= g * (1 - a) / cp; }
+            lp[r] = logf(cp);
+        }
+        ex = 1.f;
+        for (int r = 0; r < L; r++) {
+            float a = pv_or_default(E.val0, p[r]);
+            float s = pv_or_default(E.val1, p[r]);
+            float ae = a * ex;
+            fp[r] = c[r] == 1 ? ae : 1 - ae;
+            ex *= g * (1 - a) + g * a * (1 - s);
+        }
+    } else {
+        const float t1 = E.sh[0], t2 = E.sh[1], t3 = E.sh[2];
+        float ex = 1.f;
+        for (int r = 0; r < L; r++) {
+            float a = pv_or_default(E.val0, p[r]);
+            float cp;
+            if (c[r] == 1) { cp = a * ex; ex = t2 * (1 - a) + t3 * a; }
+            else { cp = 1 - a * ex; ex *= t1 * (1 - a) / cp; }
+            lp[r] = logf(cp);
+        }
+        ex = 1.f;
+        for (int r = 0; r < L; r++) {
+            float a = pv_or_default(E.val0, p[r]);
+            float ae = a * ex;
+            fp[r] = c[r] == 1 ? ae : 1 - ae;
+            ex *= (1 - a) * t1 + a * ((1 - a) * t2 + a * t3);
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int MODEL>
+__global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
+    __shared__ double red[21][4];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+    double v[21];
+#pragma unroll
+    for (int k = 0; k < 21; k++) v[k] = 0.0;
+    if (i < E.n) {
+        const int L = E.len ? (int)E.len[i] : kL;
+        float lp[kL], fp[kL];
+        eval_session<MODEL>(E, i, L, lp, fp);
+        float acc = 0.f;
+        for (int r = 0; r < L; r++) acc = acc + lp[r];
+        const float s = acc / L;                 // evaluation.cpp:36
+        v[0] = (double)s;
+        if (E.llh) E.llh[i] = s;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                v[1 + r] = (double)log2f(fp[r]);   // evaluation.cpp:62
+                v[11 + r] = 1.0;
+            }
+            if (E.logp) E.logp[i * kL + r] = r < L ? lp[r] : 0.f;
+            if (E.full) E.full[i * kL + r] = r < L ? fp[r] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 21; k++) {
+        double w = wave_sum_d(v[k]);
+        if ((tid & 63) == 0) red[k][tid >> 6] = w;
+    }
+    __syncthreads();
+    if (tid < 21) {
+        double w = 0.0;
+        for (int j = 0; j < (int)(blockDim.x >> 6); j++) w += red[tid][j];
+        E.partial[(int64_t)blockIdx.x * 21 + tid] = w;
+    }
+}
+
+// Sums block partials in a fixed order (one block, deterministic).
+__global__ void __launch_bounds__(64) eval_reduce_kernel(const double* partial, int n_blocks, double* out) {
+    const int k = threadIdx.x;
+    if (k >= 21) return;
+    double w = 0.0;
+    for (int b = 0; b < n_blocks; b++) w += partial[(int64_t)b * 21 + k];
+    out[k] = w;
+}
+
+// Fills a float array with a constant (used instead of memsets so that the
+// same code is safe inside captured graphs).
+__global__ void fill_kernel(float* p, float v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace mc
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace mc;
+
+struct mc_dataset {
+    HostDataset d;
+};
+
+struct mc_model {
+    int type = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // options
+    int reduce_mode = MC_REDUCE_SEGMENTED;
+    int item_sessions = 2048;
+    int item_window = 0;   // 0 = auto
+    int sort_by_lcr = 1;
+    int block = 256;
+    // shape
+    int T = 1, K = 10, KACC = 10;
+    int64_t N = 0, n_qd = 0;
+    bool varlen = false;
+    bool initialized = false;
+    // device buffers
+    int32_t* d_pidx = nullptr;
+    uint16_t* d_mask = nullptr;
+    uint8_t* d_len = nullptr;
+    Item* d_items = nullptr;
+    int n_items = 0;
+    int64_t max_item_sessions = 0;
+    int32_t* d_shared_idx = nullptr;
+    int64_t n_shared = 0;
+    float* d_val[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [buf][t]
+    float* d_num[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    float2* d_bd[2] = {nullptr, nullptr};
+    unsigned long long* d_gacc[2] = {nullptr, nullptr};
+    float* d_den_user[2] = {nullptr, nullptr};
+    int den_state[3] = {0, 0, 0};  // per table: 0 prior (2), 1 user-set, 2 M-step constant
+    float* d_sh_val[2] = {nullptr, nullptr};
+    float* d_sh_num[2] = {nullptr, nullptr};
+    float* d_sh_den[2] = {nullptr, nullptr};
+    double* d_sh_base = nullptr;
+    int32_t* d_sh_acc_map = nullptr;
+    long long* d_sh_partial = nullptr;
+    int cur = 0;
+    bool estep_pending = false;
+    size_t lds_bytes = 0;
+    size_t mem_bytes = 0;
+    std::vector<void*> allocations;
+    // graph cache, one slot per starting buffer parity
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_iters[2] = {0, 0};
+};
+
+namespace {
+
+const char* table_name(int type, int t) {
+    switch (type) {
+        case MC_PBM: return t == 0 ? "attractiveness" : "examination";
+        case MC_UBM: return t == 0 ? "attractiveness" : "examination";
+        case MC_CCM: return t == 0 ? "attractiveness" : "tau";
+        case MC_DBN: return t == 0 ? "attractiveness" : (t == 1 ? "satisfaction" : "gamma");
+    }
+    return "?";
+}
+
+void shape_of(int type, int* T, int* K, int* KACC) {
+    switch (type) {
+        case MC_PBM: *T = 1; *K = 10; *KACC = 10; break;
+        case MC_UBM: *T = 1; *K = 55; *KACC = 55; break;
+        case MC_DBN: *T = 2; *K = 1; *KACC = 2; break;
+        default: *T = 1; *K = 3; *KACC = 6; break;
+    }
+}
+
+template <class P>
+mc_status dalloc(mc_model* m, P** ptr, size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? MC_ERR_OOM : MC_ERR_HIP;
+    }
+    m->allocations.push_back(p);
+    m->mem_bytes += bytes;
+    *ptr = (P*)p;
+    return MC_OK;
+}
+
+#define MC_TRY(expr)                        \
+    do {                                    \
+        mc_status s_ = (expr);              \
+        if (s_ != MC_OK) return s_;         \
+    } while (0)
+
+void release(mc_model* m) {
+    for (int b = 0; b < 2; b++)
+        if (m->graph[b]) { (void)hipGraphExecDestroy(m->graph[b]); m->graph[b] = nullptr; }
+    for (void* p : m->allocations) (void)hipFree(p);
+    m->allocations.clear();
+    m->mem_bytes = 0;
+    m->initialized = false;
+}
+
+mc_status fill(mc_model* m, float* p, float v, int64_t n) {
+    if (n <= 0) return MC_OK;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, p, v, n);
+    MC_HIP(hipGetLastError());
+    return MC_OK;
+}
+
+KArgs make_args(mc_model* m, int cur) {
+    KArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.pidx = m->d_pidx;
+    A.mask = m->d_mask;
+    A.len = m->varlen ? m->d_len : nullptr;
+    A.N = m->N;
+    A.items = m->d_items;
+    A.n_items = m->n_items;
+    A.mode = m->reduce_mode;
+    for (int t = 0; t < 2; t++) {
+        int tt = t < m->T ? t : 0;
+        A.tab[t].val_in = m->d_val[cur][tt];
+        A.tab[t].val_out = m->d_val[cur ^ 1][tt];
+        A.tab[t].num_out = m->d_num[cur ^ 1][tt];
+        A.tab[t].base_den = m->d_bd[tt];
+        A.tab[t].gacc = m->d_gacc[tt];
+        A.tab[t].gaccf = (float*)m->d_gacc[tt];
+    }
+    A.sh_val_in = m->d_sh_val[cur];
+    A.sh_partial = m->d_sh_partial;
+    A.shared_idx = m->d_shared_idx;
+    A.n_shared = m->n_shared;
+    A.n_qd = m->n_qd;
+    A.n_tables = m->T;
+    A.write_num = 1;
+    A.K = m->K;
+    A.kacc = m->KACC;
+    A.sh_base = m->d_sh_base;
+    A.sh_acc_map = m->d_sh_acc_map;
+    A.sh_val_out = m->d_sh_val[cur ^ 1];
+    A.sh_num_out = m->d_sh_num[cur ^ 1];
+    A.sh_den_out = m->d_sh_den[cur ^ 1];
+    return A;
+}
+
+template <int MODEL>
+void launch_item(mc_model* m, const KArgs& A) {
+    if (m->varlen)
+        hipLaunchKernelGGL((em_item_kernel<MODEL, true>), dim3(m->n_items), dim3(m->block), m->lds_bytes, m->stream, A);
+    else
+        hipLaunchKernelGGL((em_item_kernel<MODEL, false>), dim3(m->n_items), dim3(m->block), m->lds_bytes, m->stream, A);
+}
+
+mc_status launch_estep(mc_model* m, int cur) {
+    KArgs A = make_args(m, cur);
+    switch (m->type) {
+        case MC_PBM: launch_item<kPBM>(m, A); break;
+        case MC_UBM: launch_item<kUBM>(m, A); break;
+        case MC_DBN: launch_item<kDBN>(m, A); break;
+        default: launch_item<kCCM>(m, A); break;
+    }
+    MC_HIP(hipGetLastError());
+    return MC_OK;
+}
+
+mc_status launch_mstep(mc_model* m, int cur) {
+    KArgs A = make_args(m, cur);
+    int64_t n_entries = m->reduce_mode == MC_REDUCE_ATOMIC ? m->n_qd : m->n_shared;
+    unsigned grid = (unsigned)(m->K + (n_entries + 255) / 256);
+    switch (m->type) {
+        case MC_PBM: hipLaunchKernelGGL(em_merge_kernel<kPBM>, dim3(grid), dim3(256), 0, m->stream, A); break;
+        case MC_UBM: hipLaunchKernelGGL(em_merge_kernel<kUBM>, dim3(grid), dim3(256), 0, m->stream, A); break;
+        case MC_DBN: hipLaunchKernelGGL(em_merge_kernel<kDBN>, dim3(grid), dim3(256), 0, m->stream, A); break;
+        default: hipLaunchKernelGGL(em_merge_kernel<kCCM>, dim3(grid), dim3(256), 0, m->stream, A); break;
+    }
+    MC_HIP(hipGetLastError());
+    return MC_OK;
+}
+
+// Captures n_iter iterations starting from buffer parity `cur` into a graph
+// (cached per parity).
+mc_status get_graph(mc_model* m, int n_iter, int cur, hipGraphExec_t* out) {
+    if (m->graph[cur] && m->graph_iters[cur] == n_iter) { *out = m->graph[cur]; return MC_OK; }
+    if (m->graph[cur]) { MC_HIP(hipGraphExecDestroy(m->graph[cur])); m->graph[cur] = nullptr; }
+    hipGraph_t g;
+    MC_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    int c = cur;
+    mc_status st = MC_OK;
+    for (int i = 0; i < n_iter && st == MC_OK; i++) {
+        st = launch_estep(m, c);
+        if (st == MC_OK) st = launch_mstep(m, c);
+        c ^= 1;
+    }
+    hipError_t e = hipStreamEndCapture(m->stream, &g);
+    if (st != MC_OK) return st;
+    MC_HIP(e);
+    e = hipGraphInstantiate(&m->graph[cur], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MC_HIP(e);
+    m->graph_iters[cur] = n_iter;
+    *out = m->graph[cur];
+    return MC_OK;
+}
+
+// Largest per-qd accumulator window that keeps two workgroups per CU.
+int default_window(int T) { return (48 * 1024) / (T * 8); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mc_last_error(void) { return g_last_error.c_str(); }
+
+mc_status mc_dataset_parse(const char* path, int max_sessions, float test_share, mc_dataset** out) {
+    MC_REQUIRE(path && out, "null argument");
+    MC_REQUIRE(max_sessions == -1 || max_sessions > 0, "max_sessions must be -1 or positive");
+    std::vector<int32_t> q, d;
+    std::vector<uint8_t> c;
+    std::string err;
+    try {
+        if (!parse_raw_log(path, max_sessions, &q, &d, &c, &err)) {
+            set_error(err);
+            return err.rfind("cannot open", 0) == 0 ? MC_ERR_IO : MC_ERR_INVALID_ARG;
+        }
+        mc_dataset* ds = new mc_dataset;
+        if (!prepare_dataset(ds->d, q, d, c, test_share, &err)) {
+            delete ds;
+            set_error(err);
+            return MC_ERR_INVALID_ARG;
+        }
+        *out = ds;
+    } catch (std::bad_alloc&) {
+        set_error("out of host memory while preparing the dataset");
+        return MC_ERR_OOM;
+    }
+    return MC_OK;
+}
+
+mc_status mc_dataset_from_sessions(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
+                                   float test_share, mc_dataset** out) {
+    MC_REQUIRE(out && n > 0 && query && docs && clicks, "invalid argument");
+    MC_REQUIRE(n <= INT32_MAX, "too many sessions");
+    std::string err;
+    try {
+        std::vector<int32_t> q(query, query + n), d(docs, docs + n * kMaxSerp);
+        std::vector<uint8_t> c(clicks, clicks + n * kMaxSerp);
+        mc_dataset* ds = new mc_dataset;
+        if (!prepare_dataset(ds->d, q, d, c, test_share, &err)) {
+            delete ds;
+            set_error(err);
+            return MC_ERR_INVALID_ARG;
+        }
+        *out = ds;
+    } catch (std::bad_alloc&) {
+        set_error("out of host memory while preparing the dataset");
+        return MC_ERR_OOM;
+    }
+    return MC_OK;
+}
+
+mc_status mc_dataset_sizes(const mc_dataset* ds, int64_t* n_parsed, int64_t* n_train, int64_t* n_test,
+                           int64_t* n_qd) {
+    MC_REQUIRE(ds, "null dataset");
+    if (n_parsed) *n_parsed = ds->d.n_parsed;
+    if (n_train) *n_train = (int64_t)ds->d.train_query.size();
+    if (n_test) *n_test = (int64_t)ds->d.test_query.size();
+    if (n_qd) *n_qd = ds->d.n_qd;
+    return MC_OK;
+}
+
+mc_status mc_dataset_export(const mc_dataset* ds, int which, int32_t* query, int32_t* docs, uint8_t* clicks,
+                            int32_t* pidx) {
+    MC_REQUIRE(ds && (which == 0 || which == 1), "invalid argument");
+    const HostDataset& d = ds->d;
+    const auto& Q = which == 0 ? d.train_query : d.test_query;
+    const auto& D = which == 0 ? d.train_docs : d.test_docs;
+    const auto& C = which == 0 ? d.train_clicks : d.test_clicks;
+    const auto& P = which == 0 ? d.train_pidx : d.test_pidx;
+    if (query) std::copy(Q.begin(), Q.end(), query);
+    if (docs) std::copy(D.begin(), D.end(), docs);
+    if (clicks) std::copy(C.begin(), C.end(), clicks);
+    if (pidx) std::copy(P.begin(), P.end(), pidx);
+    return MC_OK;
+}
+
+void mc_dataset_free(mc_dataset* ds) { delete ds; }
+
+mc_status mc_model_create(int model_type, int device, mc_model** out) {
+    MC_REQUIRE(out, "null output pointer");
+    MC_REQUIRE(model_type >= 0 && model_type <= 3, "model_type must be 0 (PBM), 1 (CCM), 2 (DBN) or 3 (UBM)");
+    int n_dev = 0;
+    MC_HIP(hipGetDeviceCount(&n_dev));
+    MC_REQUIRE(device >= 0 && device < n_dev, "no such HIP device");
+    MC_HIP(hipSetDevice(device));
+    mc_model* m = new mc_model;
+    m->type = model_type;
+    m->device = device;
+    shape_of(model_type, &m->T, &m->K, &m->KACC);
+    hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete m;
+        set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        return MC_ERR_HIP;
+    }
+    *out = m;
+    return MC_OK;
+}
+
+mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
+    MC_REQUIRE(m, "null model");
+    MC_REQUIRE(!m->initialized, "options must be set before mc_init_parameters");
+    switch (option) {
+        case MC_OPT_REDUCE_MODE:
+            MC_REQUIRE(value == MC_REDUCE_SEGMENTED || value == MC_REDUCE_ATOMIC, "bad reduce mode");
+            m->reduce_mode = (int)value;
+            break;
+        case MC_OPT_ITEM_SESSIONS:
+            MC_REQUIRE(value >= 64 && value <= (1 << 20), "item sessions out of range [64, 2^20]");
+            m->item_sessions = (int)value;
+            break;
+        case MC_OPT_ITEM_WINDOW:
+            MC_REQUIRE(value >= 0 && value <= 16384, "item window out of range [0, 16384]");
+            m->item_window = (int)value;
+            break;
+        case MC_OPT_SORT_BY_LCR:
+            m->sort_by_lcr = value != 0;
+            break;
+        case MC_OPT_BLOCK:
+            MC_REQUIRE(value == 64 || value == 128 || value == 256 || value == 512, "block must be 64/128/256/512");
+            m->block = (int)value;
+            break;
+        default:
+            set_error("unknown option");
+            return MC_ERR_INVALID_ARG;
+    }
+    return MC_OK;
+}
+
+mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const int32_t* pidx,
+                             const uint8_t* clicks, const uint8_t* len, int64_t n_qd) {
+    MC_REQUIRE(m && query && pidx && clicks, "null argument");
+    MC_REQUIRE(n >= 1 && n <= INT32_MAX - 1, "session count out of range");
+    MC_REQUIRE(n_qd >= 1 && n_qd <= INT32_MAX - 1, "n_qd out of range");
+    MC_HIP(hipSetDevice(m->device));
+    if (m->initialized) release(m);
+    const int T = m->T, K = m->K, KACC = m->KACC;
+    m->N = n;
+    m->n_qd = n_qd;
+    m->varlen = len != nullptr;
+
+    // ---- validate and summarise the sessions ---------------------------------
+    std::vector<uint8_t> L((size_t)n, (uint8_t)kMaxSerp);
+    std::vector<uint16_t> mask((size_t)n, 0);
+    std::vector<uint8_t> lcr((size_t)n, 0);
+    for (int64_t s = 0; s < n; s++) {
+        int l = kMaxSerp;
+        if (len) {
+            l = len[s];
+            MC_REQUIRE(l >= 1 && l <= kMaxSerp, "len must be in [1, 10] (session " + std::to_string(s) + ")");
+        }
+        L[s] = (uint8_t)l;
+        uint16_t mk = 0;
+        for (int r = 0; r < l; r++) {
+            const int32_t p = pidx[s * kMaxSerp + r];
+            MC_REQUIRE(p >= 0 && p < n_qd, "pidx out of [0, n_qd) at session " + std::to_string(s));
+            const uint8_t c = clicks[s * kMaxSerp + r];
+            MC_REQUIRE(c <= 1, "clicks must be 0 or 1");
+            if (c) mk |= (uint16_t)(1u << r);
+        }
+        mask[s] = mk;
+        lcr[s] = mk ? (uint8_t)(31 - __builtin_clz((unsigned)mk)) : (uint8_t)kMaxSerp;
+    }
+
+    // ---- constants: per-qd (base numerator, denominator), shared bases -------
+    // occ/clicked counts per entry; see the session functions in mc_kernels.h
+    // for which counts are parameter-independent.
+    std::vector<int64_t> occ((size_t)n_qd, 0), clk((size_t)n_qd, 0);
+    for (int64_t s = 0; s < n; s++)
+        for (int r = 0; r < L[s]; r++) {
+            const int32_t p = pidx[s * kMaxSerp + r];
+            occ[p]++;
+            if ((mask[s] >> r) & 1u) clk[p]++;
+        }
+    std::vector<float2> bd0((size_t)n_qd), bd1;
+    for (int64_t p = 0; p < n_qd; p++) {
+        const double c = (double)clk[p], o = (double)occ[p];
+        switch (m->type) {
+            case MC_CCM: bd0[p] = make_float2((float)(1.0 + c), (float)(2.0 + o + c)); break;
+            default: bd0[p] = make_float2((float)(1.0 + c), (float)(2.0 + o)); break;
+        }
+    }
+    if (m->type == MC_DBN) {
+        bd1.resize((size_t)n_qd);
+        for (int64_t p = 0; p < n_qd; p++) {
+            const double c = (double)clk[p], u = (double)(occ[p] - clk[p]);
+            bd1[p] = make_float2((float)(1.0 + u), (float)(2.0 + c + 2.0 * u));
+        }
+    }
+    std::vector<double> sh_base(2 * (size_t)K, 0.0);
+    std::vector<int32_t> sh_map(2 * (size_t)K, -1);
+    switch (m->type) {
+        case MC_PBM:
+        case MC_UBM: {
+            // w[k]: sessions whose temporary slot k is written (by the last
+            // rank writing it, see ubm_session), wc[k]: of those, written by
+            // a clicked rank with (1,1).  Unwritten slots keep (1,2).
+            std::vector<int64_t> w((size_t)K, 0), wc((size_t)K, 0);
+            for (int64_t s = 0; s < n; s++) {
+                int last = kMaxSerp - 1;
+                int slot[kMaxSerp];
+                for (int r = 0; r < L[s]; r++) {
+                    slot[r] = m->type == MC_PBM ? r : r * (r + 1) / 2 + last;
+                    if ((mask[s] >> r) & 1u) last = r;
+                }
+                for (int r = 0; r < L[s]; r++) {
+                    bool last_writer = true;
+                    for (int q = r + 1; q < L[s]; q++) last_writer = last_writer && slot[q] != slot[r];
+                    if (!last_writer) continue;
+                    w[slot[r]]++;
+                    if ((mask[s] >> r) & 1u) wc[slot[r]]++;
+                }
+            }
+            for (int k = 0; k < K; k++) {
+                const double ph = (double)(n - w[k]);
+                sh_base[2 * k] = 1.0 + (double)wc[k] + ph;
+                sh_base[2 * k + 1] = 2.0 + (double)w[k] + 2.0 * ph;
+                sh_map[2 * k] = k;
+            }
+            break;
+        }
+        case MC_DBN: {
+            int64_t n0 = 0;
+            for (int64_t s = 0; s < n; s++) n0 += std::min<int64_t>(lcr[s], L[s]);
+            sh_base[0] = 1.0 + (double)n0;
+            sh_base[1] = 2.0 + (double)n0;
+            sh_map[0] = 0;
+            sh_map[1] = 1;
+            break;
+        }
+        default: {
+            int64_t n1 = 0;
+            for (int64_t s = 0; s < n; s++)
+                for (int r = 0; r < std::min<int>(lcr[s], L[s]); r++)
+                    if (!((mask[s] >> r) & 1u)) n1++;
+            sh_base[0] = 1.0 + (double)n1;
+            sh_base[1] = 2.0 + (double)n1;
+            for (int k = 1; k < 3; k++) { sh_base[2 * k] = 1.0; sh_base[2 * k + 1] = 2.0; }
+            for (int k = 0; k < 3; k++) { sh_map[2 * k] = 2 * k; sh_map[2 * k + 1] = 2 * k + 1; }
+            break;
+        }
+    }
+
+    // ---- work items -----------------------------------------------------------
+    const int W = m->item_window > 0 ? m->item_window : default_window(T);
+    const int S = m->item_sessions;
+    std::vector<Item> items;
+    std::vector<int32_t> shared_idx;
+    std::vector<int64_t> order((size_t)n);   // device position -> input session
+    std::iota(order.begin(), order.end(), 0);
+    bool structured = m->reduce_mode == MC_REDUCE_SEGMENTED;
+    if (structured) {
+        // Query groups: runs of equal query ids.  The segmented reduction needs
+        // each group's entries in a range [lo, hi) disjoint from and after the
+        // previous group's, which first-occurrence indexing of a query-sorted
+        // train set guarantees (dataset.cpp:314-339).
+        struct Group { int64_t s0, s1; int32_t lo, hi; };
+        std::vector<Group> groups;
+        int64_t prev_hi = 0;
+        for (int64_t s = 0; s < n && structured;) {
+            int64_t e = s;
+            int32_t lo = INT32_MAX, hi = -1;
+            while (e < n && query[e] == query[s]) {
+                for (int r = 0; r < L[e]; r++) {
+                    lo = std::min(lo, pidx[e * kMaxSerp + r]);
+                    hi = std::max(hi, pidx[e * kMaxSerp + r] + 1);
+                }
+                e++;
+            }
+            if (lo < prev_hi) structured = false;
+            prev_hi = hi;
+            groups.push_back(Group{s, e, lo, hi});
+            s = e;
+        }
+        if (structured) {
+            Item cur{0, 0, 0, 0, kItemInterior, 0};
+            bool open = false;
+            auto close = [&]() { if (open) { items.push_back(cur); open = false; } };
+            for (const Group& g : groups) {
+                const int64_t gs = g.s1 - g.s0;
+                const int64_t gw = g.hi - g.lo;
+                if (gs <= S && gw <= W) {
+                    if (open && (cur.s1 - cur.s0) + gs <= S && (g.hi - cur.base) <= W) {
+                        cur.s1 = (int32_t)g.s1;
+                        cur.width = g.hi - cur.base;
+                    } else {
+                        close();
+                        cur = Item{(int32_t)g.s0, (int32_t)g.s1, g.lo, (int32_t)gw, kItemInterior, 0};
+                        open = true;
+                    }
+                } else {
+                    close();
+                    const int64_t n_slices = (gs + S - 1) / S;
+                    for (int64_t k = 0; k < n_slices; k++) {
+                        const int64_t a = g.s0 + gs * k / n_slices, b = g.s0 + gs * (k + 1) / n_slices;
+                        items.push_back(Item{(int32_t)a, (int32_t)b, g.lo, gw <= W ? (int32_t)gw : 0, 0, 0});
+                    }
+                    for (int32_t p = g.lo; p < g.hi; p++) shared_idx.push_back(p);
+                }
+            }
+            close();
+        }
+    }
+    if (!structured) {
+        // Float atomics over uniform chunks; the merge pass finalises every entry.
+        m->reduce_mode = MC_REDUCE_ATOMIC;
+        items.clear();
+        shared_idx.clear();
+        for (int64_t a = 0; a < n; a += S) items.push_back(Item{(int32_t)a, (int32_t)std::min<int64_t>(n, a + S), 0, 0, 0, 0});
+    }
+    // Every session of an item with a window must index inside that window,
+    // and items must tile [0, n) in order: the kernel relies on both.
+    {
+        int64_t expect = 0;
+        for (const Item& it : items) {
+            MC_REQUIRE(it.s0 == expect && it.s1 > it.s0 && it.s1 <= n, "internal: work items do not tile the sessions");
+            expect = it.s1;
+            if (it.width > 0) {
+                MC_REQUIRE(it.base >= 0 && (int64_t)it.base + it.width <= n_qd && it.width <= W,
+                           "internal: item window out of range");
+                for (int64_t s = it.s0; s < it.s1; s++)
+                    for (int r = 0; r < L[s]; r++) {
+                        const int32_t p = pidx[s * kMaxSerp + r];
+                        MC_REQUIRE(p >= it.base && p < it.base + it.width, "internal: session outside its item window");
+                    }
+            }
+        }
+        MC_REQUIRE(expect == n, "internal: work items do not cover the sessions");
+    }
+    m->n_items = (int)items.size();
+    m->n_shared = (int64_t)shared_idx.size();
+    m->max_item_sessions = 0;
+    for (const Item& it : items) m->max_item_sessions = std::max<int64_t>(m->max_item_sessions, it.s1 - it.s0);
+    int max_width = 0;
+    for (const Item& it : items) max_width = std::max(max_width, (int)it.width);
+    m->lds_bytes = m->reduce_mode == MC_REDUCE_SEGMENTED ? (size_t)max_width * T * 8 : 0;
+
+    // Reorder the sessions of each item by last-click rank (wavefront-uniform
+    // chain lengths); the per-entry sums do not depend on the order.
+    if (m->sort_by_lcr) {
+        for (const Item& it : items) {
+            std::stable_sort(order.begin() + it.s0, order.begin() + it.s1,
+                             [&](int64_t a, int64_t b) { return lcr[a] < lcr[b]; });
+        }
+    }
+
+    // ---- device layout: rank-major pidx, click masks, lengths ---------------
+    std::vector<int32_t> h_pidx((size_t)n * kMaxSerp);
+    std::vector<uint16_t> h_mask((size_t)n);
+    std::vector<uint8_t> h_len((size_t)n);
+    for (int64_t j = 0; j < n; j++) {
+        const int64_t s = order[j];
+        for (int r = 0; r < kMaxSerp; r++)
+            h_pidx[(size_t)r * n + j] = r < L[s] ? pidx[s * kMaxSerp + r] : 0;
+        h_mask[j] = mask[s];
+        h_len[j] = L[s];
+    }
+
+    MC_TRY(dalloc(m, &m->d_pidx, h_pidx.size() * 4));
+    MC_TRY(dalloc(m, &m->d_mask, h_mask.size() * 2));
+    if (m->varlen) MC_TRY(dalloc(m, &m->d_len, h_len.size()));
+    MC_TRY(dalloc(m, &m->d_items, items.size() * sizeof(Item)));
+    MC_TRY(dalloc(m, &m->d_shared_idx, shared_idx.size() * 4));
+    for (int t = 0; t < T; t++) {
+        for (int b = 0; b < 2; b++) {
+            MC_TRY(dalloc(m, &m->d_val[b][t], (size_t)n_qd * 4));
+            MC_TRY(dalloc(m, &m->d_num[b][t], (size_t)n_qd * 4));
+        }
+        MC_TRY(dalloc(m, &m->d_bd[t], (size_t)n_qd * 8));
+        MC_TRY(dalloc(m, &m->d_gacc[t], (size_t)n_qd * 8));
+    }
+    for (int b = 0; b < 2; b++) {
+        MC_TRY(dalloc(m, &m->d_sh_val[b], (size_t)K * 4));
+        MC_TRY(dalloc(m, &m->d_sh_num[b], (size_t)K * 4));
+        MC_TRY(dalloc(m, &m->d_sh_den[b], (size_t)K * 4));
+    }
+    MC_TRY(dalloc(m, &m->d_sh_base, sh_base.size() * 8));
+    MC_TRY(dalloc(m, &m->d_sh_acc_map, sh_map.size() * 4));
+    MC_TRY(dalloc(m, &m->d_sh_partial, (size_t)KACC * items.size() * 8));
+
+    MC_HIP(hipMemcpyAsync(m->d_pidx, h_pidx.data(), h_pidx.size() * 4, hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_mask, h_mask.data(), h_mask.size() * 2, hipMemcpyHostToDevice, m->stream));
+    if (m->varlen)
+        MC_HIP(hipMemcpyAsync(m->d_len, h_len.data(), h_len.size(), hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_items, items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice, m->stream));
+    if (!shared_idx.empty())
+        MC_HIP(hipMemcpyAsync(m->d_shared_idx, shared_idx.data(), shared_idx.size() * 4, hipMemcpyHostToDevice,
+                              m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_bd[0], bd0.data(), bd0.size() * 8, hipMemcpyHostToDevice, m->stream));
+    if (T > 1) MC_HIP(hipMemcpyAsync(m->d_bd[1], bd1.data(), bd1.size() * 8, hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_sh_base, sh_base.data(), sh_base.size() * 8, hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_sh_acc_map, sh_map.data(), sh_map.size() * 4, hipMemcpyHostToDevice, m->stream));
+    for (int t = 0; t < T; t++) {
+        MC_HIP(hipMemsetAsync(m->d_gacc[t], 0, (size_t)n_qd * 8, m->stream));
+        for (int b = 0; b < 2; b++) {
+            MC_TRY(fill(m, m->d_val[b][t], 0.5f, n_qd));
+            MC_TRY(fill(m, m->d_num[b][t], 1.f, n_qd));
+        }
+        m->den_state[t] = 0;
+    }
+    for (int b = 0; b < 2; b++) {
+        MC_TRY(fill(m, m->d_sh_val[b], 0.5f, K));
+        MC_TRY(fill(m, m->d_sh_num[b], 1.f, K));
+        MC_TRY(fill(m, m->d_sh_den[b], 2.f, K));
+    }
+    m->den_state[T] = 0;
+    MC_HIP(hipStreamSynchronize(m->stream));
+    m->cur = 0;
+    m->estep_pending = false;
+    m->initialized = true;
+    return MC_OK;
+}
+
+mc_status mc_init_from_dataset(mc_model* m, const mc_dataset* ds) {
+    MC_REQUIRE(m && ds, "null argument");
+    const HostDataset& d = ds->d;
+    return mc_init_parameters(m, (int64_t)d.train_query.size(), d.train_query.data(), d.train_pidx.data(),
+                              d.train_clicks.data(), nullptr, d.n_qd);
+}
+
+mc_status mc_process_session(mc_model* m) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(!m->estep_pending, "E-step already pending; call mc_update_parameters");
+    MC_HIP(hipSetDevice(m->device));
+    MC_TRY(launch_estep(m, m->cur));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    m->estep_pending = true;
+    return MC_OK;
+}
+
+mc_status mc_update_parameters(mc_model* m) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(m->estep_pending, "no pending E-step; call mc_process_session first");
+    MC_HIP(hipSetDevice(m->device));
+    MC_TRY(launch_mstep(m, m->cur));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    m->cur ^= 1;
+    m->estep_pending = false;
+    for (int t = 0; t <= m->T; t++) m->den_state[t] = 2;
+    return MC_OK;
+}
+
+mc_status mc_em_iterations(mc_model* m, int n_iter, int use_graph) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(!m->estep_pending, "an E-step is pending");
+    MC_REQUIRE(n_iter >= 0, "n_iter must be >= 0");
+    if (n_iter == 0) return MC_OK;
+    MC_HIP(hipSetDevice(m->device));
+    if (use_graph) {
+        hipGraphExec_t g = nullptr;
+        MC_TRY(get_graph(m, n_iter, m->cur, &g));
+        MC_HIP(hipGraphLaunch(g, m->stream));
+    } else {
+        int c = m->cur;
+        for (int i = 0; i < n_iter; i++) {
+            MC_TRY(launch_estep(m, c));
+            MC_TRY(launch_mstep(m, c));
+            c ^= 1;
+        }
+    }
+    MC_HIP(hipStreamSynchronize(m->stream));
+    m->cur ^= (n_iter & 1);
+    for (int t = 0; t <= m->T; t++) m->den_state[t] = 2;
+    return MC_OK;
+}
+
+mc_status mc_em_iterations_timed(mc_model* m, int n_iter, double* ms_total, double* ms_estep_avg,
+                                 double* ms_mstep_avg) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(!m->estep_pending, "an E-step is pending");
+    MC_REQUIRE(n_iter >= 1, "n_iter must be >= 1");
+    MC_HIP(hipSetDevice(m->device));
+    std::vector<hipEvent_t> ev((size_t)(3 * n_iter + 1));
+    for (auto& e : ev) MC_HIP(hipEventCreate(&e));
+    int c = m->cur;
+    MC_HIP(hipEventRecord(ev[0], m->stream));
+    for (int i = 0; i < n_iter; i++) {
+        MC_HIP(hipEventRecord(ev[3 * i + 1], m->stream));
+        MC_TRY(launch_estep(m, c));
+        MC_HIP(hipEventRecord(ev[3 * i + 2], m->stream));
+        MC_TRY(launch_mstep(m, c));
+        MC_HIP(hipEventRecord(ev[3 * i + 3], m->stream));
+        c ^= 1;
+    }
+    MC_HIP(hipStreamSynchronize(m->stream));
+    double te = 0, tm = 0;
+    for (int i = 0; i < n_iter; i++) {
+        float a = 0, b = 0;
+        MC_HIP(hipEventElapsedTime(&a, ev[3 * i + 1], ev[3 * i + 2]));
+        MC_HIP(hipEventElapsedTime(&b, ev[3 * i + 2], ev[3 * i + 3]));
+        te += a;
+        tm += b;
+    }
+    float tot = 0;
+    MC_HIP(hipEventElapsedTime(&tot, ev[0], ev[3 * n_iter]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (ms_total) *ms_total = tot;
+    if (ms_estep_avg) *ms_estep_avg = te / n_iter;
+    if (ms_mstep_avg) *ms_mstep_avg = tm / n_iter;
+    m->cur ^= (n_iter & 1);
+    for (int t = 0; t <= m->T; t++) m->den_state[t] = 2;
+    return MC_OK;
+}
+
+mc_status mc_em_graph_timed(mc_model* m, int n_iter, double* ms_total) {
+    MC_REQUIRE(m && m->initialized && n_iter >= 1, "invalid argument");
+    MC_REQUIRE(!m->estep_pending, "an E-step is pending");
+    MC_HIP(hipSetDevice(m->device));
+    // One warm replay, then the timed replay (graphs built outside the timing).
+    MC_TRY(mc_em_iterations(m, n_iter, 1));
+    hipGraphExec_t g = nullptr;
+    MC_TRY(get_graph(m, n_iter, m->cur, &g));
+    hipEvent_t a, b;
+    MC_HIP(hipEventCreate(&a));
+    MC_HIP(hipEventCreate(&b));
+    MC_HIP(hipEventRecord(a, m->stream));
+    MC_HIP(hipGraphLaunch(g, m->stream));
+    MC_HIP(hipEventRecord(b, m->stream));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    float ms = 0;
+    MC_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    m->cur ^= (n_iter & 1);
+    for (int t = 0; t <= m->T; t++) m->den_state[t] = 2;
+    if (ms_total) *ms_total = ms;
+    return MC_OK;
+}
+
+mc_status mc_synchronize(mc_model* m) {
+    MC_REQUIRE(m, "null model");
+    MC_HIP(hipStreamSynchronize(m->stream));
+    return MC_OK;
+}
+
+mc_status mc_get_n_tables(const mc_model* m, int* n_tables) {
+    MC_REQUIRE(m && n_tables, "null argument");
+    *n_tables = m->T + 1;
+    return MC_OK;
+}
+
+mc_status mc_get_table_info(const mc_model* m, int t, const char** name, int* is_public, int64_t* size) {
+    MC_REQUIRE(m && t >= 0 && t <= m->T, "bad table index");
+    if (name) *name = table_name(m->type, t);
+    if (is_public) *is_public = t == m->T;
+    if (size) *size = t == m->T ? m->K : m->n_qd;
+    return MC_OK;
+}
+
+mc_status mc_get_parameters(mc_model* m, int t, float* num, float* den) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(t >= 0 && t <= m->T, "bad table index");
+    MC_HIP(hipSetDevice(m->device));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    const bool shared = t == m->T;
+    const int64_t n = shared ? m->K : m->n_qd;
+    if (num) MC_HIP(hipMemcpy(num, shared ? m->d_sh_num[m->cur] : m->d_num[m->cur][t], n * 4, hipMemcpyDeviceToHost));
+    if (den) {
+        if (shared) {
+            MC_HIP(hipMemcpy(den, m->d_sh_den[m->cur], n * 4, hipMemcpyDeviceToHost));
+        } else if (m->den_state[t] == 0) {
+            std::fill(den, den + n, 2.f);
+        } else if (m->den_state[t] == 1) {
+            MC_HIP(hipMemcpy(den, m->d_den_user[t], n * 4, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float2> bd((size_t)n);
+            MC_HIP(hipMemcpy(bd.data(), m->d_bd[t], n * 8, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; i++) den[i] = bd[i].y;
+        }
+    }
+    return MC_OK;
+}
+
+mc_status mc_set_parameters(mc_model* m, int t, const float* num, const float* den) {
+    MC_REQUIRE(m && m->initialized && num && den, "invalid argument");
+    MC_REQUIRE(t >= 0 && t <= m->T, "bad table index");
+    MC_REQUIRE(!m->estep_pending, "an E-step is pending");
+    MC_HIP(hipSetDevice(m->device));
+    const bool shared = t == m->T;
+    const int64_t n = shared ? m->K : m->n_qd;
+    std::vector<float> val((size_t)n);
+    for (int64_t i = 0; i < n; i++) {   // Param::value, param.cu:75-87
+        const float v = num[i] / den[i];
+        val[i] = v < kClampMax ? v : kClampMax;
+    }
+    float* dv = shared ? m->d_sh_val[m->cur] : m->d_val[m->cur][t];
+    float* dn = shared ? m->d_sh_num[m->cur] : m->d_num[m->cur][t];
+    MC_HIP(hipMemcpy(dv, val.data(), n * 4, hipMemcpyHostToDevice));
+    MC_HIP(hipMemcpy(dn, num, n * 4, hipMemcpyHostToDevice));
+    if (shared) {
+        MC_HIP(hipMemcpy(m->d_sh_den[m->cur], den, n * 4, hipMemcpyHostToDevice));
+    } else {
+        if (!m->d_den_user[t]) MC_TRY(dalloc(m, &m->d_den_user[t], (size_t)n * 4));
+        MC_HIP(hipMemcpy(m->d_den_user[t], den, n * 4, hipMemcpyHostToDevice));
+        m->den_state[t] = 1;
+    }
+    return MC_OK;
+}
+
+mc_status mc_reset_parameters(mc_model* m) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_HIP(hipSetDevice(m->device));
+    for (int t = 0; t < m->T; t++) {
+        MC_TRY(fill(m, m->d_val[m->cur][t], 0.5f, m->n_qd));
+        MC_TRY(fill(m, m->d_num[m->cur][t], 1.f, m->n_qd));
+        m->den_state[t] = 0;
+    }
+    MC_TRY(fill(m, m->d_sh_val[m->cur], 0.5f, m->K));
+    MC_TRY(fill(m, m->d_sh_num[m->cur], 1.f, m->K));
+    MC_TRY(fill(m, m->d_sh_den[m->cur], 2.f, m->K));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    m->estep_pending = false;
+    return MC_OK;
+}
+
+mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                      float* logp, float* full, float* llh, double* llh_sum, double* ppl_log2_sum,
+                      int64_t* rank_count) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(n >= 1 && pidx && clicks, "invalid argument");
+    MC_HIP(hipSetDevice(m->device));
+    for (int64_t i = 0; i < n; i++) {
+        const int l = len ? len[i] : kMaxSerp;
+        MC_REQUIRE(l >= 1 && l <= kMaxSerp, "len must be in [1, 10]");
+        for (int r = 0; r < l; r++) {
+            const int32_t p = pidx[i * kMaxSerp + r];
+            MC_REQUIRE(p >= -1 && p < m->n_qd, "test pidx out of [-1, n_qd)");
+            MC_REQUIRE(clicks[i * kMaxSerp + r] <= 1, "clicks must be 0 or 1");
+        }
+    }
+    // Temporary device buffers for this call.
+    const int64_t nb = (n + 255) / 256;
+    std::vector<void*> tmp;
+    auto talloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+        tmp.push_back(p);
+        return p;
+    };
+    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
+    int32_t* d_p = (int32_t*)talloc((size_t)n * kMaxSerp * 4);
+    uint8_t* d_c = (uint8_t*)talloc((size_t)n * kMaxSerp);
+    uint8_t* d_l = len ? (uint8_t*)talloc((size_t)n) : nullptr;
+    float* d_lp = logp ? (float*)talloc((size_t)n * kMaxSerp * 4) : nullptr;
+    float* d_fp = full ? (float*)talloc((size_t)n * kMaxSerp * 4) : nullptr;
+    float* d_llh = llh ? (float*)talloc((size_t)n * 4) : nullptr;
+    double* d_part = (double*)talloc((size_t)nb * 21 * 8);
+    double* d_out = (double*)talloc(21 * 8);
+    if (!d_p || !d_c || (len && !d_l) || (logp && !d_lp) || (full && !d_fp) || (llh && !d_llh) || !d_part || !d_out) {
+        cleanup();
+        set_error("device allocation failed in mc_evaluate");
+        return MC_ERR_OOM;
+    }
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess && x != hipSuccess) e = x; };
+    chk(hipMemcpyAsync(d_p, pidx, (size_t)n * kMaxSerp * 4, hipMemcpyHostToDevice, m->stream));
+    chk(hipMemcpyAsync(d_c, clicks, (size_t)n * kMaxSerp, hipMemcpyHostToDevice, m->stream));
+    if (len) chk(hipMemcpyAsync(d_l, len, (size_t)n, hipMemcpyHostToDevice, m->stream));
+    EvalArgs E;
+    E.pidx = d_p;
+    E.clicks = d_c;
+    E.len = d_l;
+    E.n = n;
+    E.val0 = m->d_val[m->cur][0];
+    E.val1 = m->T > 1 ? m->d_val[m->cur][1] : m->d_val[m->cur][0];
+    E.sh = m->d_sh_val[m->cur];
+    E.logp = d_lp;
+    E.full = d_fp;
+    E.llh = d_llh;
+    E.partial = d_part;
+    switch (m->type) {
+        case MC_PBM: hipLaunchKernelGGL(eval_kernel<kPBM>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
+        case MC_UBM: hipLaunchKernelGGL(eval_kernel<kUBM>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
+        case MC_DBN: hipLaunchKernelGGL(eval_kernel<kDBN>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
+        default: hipLaunchKernelGGL(eval_kernel<kCCM>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
+    }
+    chk(hipGetLastError());
+    hipLaunchKernelGGL(eval_reduce_kernel, dim3(1), dim3(64), 0, m->stream, d_part, (int)nb, d_out);
+    chk(hipGetLastError());
+    double out[21];
+    chk(hipMemcpyAsync(out, d_out, 21 * 8, hipMemcpyDeviceToHost, m->stream));
+    if (logp) chk(hipMemcpyAsync(logp, d_lp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
+    if (full) chk(hipMemcpyAsync(full, d_fp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
+    if (llh) chk(hipMemcpyAsync(llh, d_llh, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream));
+    chk(hipStreamSynchronize(m->stream));
+    cleanup();
+    if (e != hipSuccess) {
+        set_error(std::string("mc_evaluate: ") + hipGetErrorString(e));
+        return MC_ERR_HIP;
+    }
+    if (llh_sum) *llh_sum = out[0];
+    for (int r = 0; r < kMaxSerp; r++) {
+        if (ppl_log2_sum) ppl_log2_sum[r] = out[1 + r];
+        if (rank_count) rank_count[r] = (int64_t)out[11 + r];
+    }
+    return MC_OK;
+}
+
+mc_status mc_get_memory_usage(const mc_model* m, size_t* bytes) {
+    MC_REQUIRE(m && bytes, "null argument");
+    *bytes = m->mem_bytes;
+    return MC_OK;
+}
+
+mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, size_t* bytes) {
+    MC_REQUIRE(bytes && model_type >= 0 && model_type <= 3 && n >= 0 && n_qd >= 0, "invalid argument");
+    int T, K, KACC;
+    shape_of(model_type, &T, &K, &KACC);
+    // sessions: 40 B pidx + 2 B mask; per table entry: 2 values, 2 numerators,
+    // (base, den) and an 8-byte accumulator; shared partials per ~1k sessions.
+    size_t b = (size_t)n * 42 + (size_t)n_qd * T * (16 + 8 + 8) + (size_t)K * 32;
+    b += (size_t)KACC * 8 * (size_t)(n / 1024 + 1) + (size_t)(n / 1024 + 1) * sizeof(Item);
+    *bytes = b;
+    return MC_OK;
+}
+
+mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
+                             int64_t* max_item_sessions) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    if (n_items) *n_items = m->n_items;
+    if (n_shared_entries) *n_shared_entries = m->n_shared;
+    if (max_item_sessions) *max_item_sessions = m->max_item_sessions;
+    return MC_OK;
+}
+
+void mc_model_destroy(mc_model* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    (void)hipStreamSynchronize(m->stream);
+    release(m);
+    (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+}  // extern "C"

@@ -1,0 +1,388 @@
+// mc_kernels.h -- CDNA4 device code of the EM engine (PBM, UBM, CCM, DBN).
+//
+// One workgroup processes one work item: a contiguous run of training
+// sessions (sorted by query, so the query-document entries it touches form a
+// contiguous window).  Each lane runs one session at a time; the expected
+// counts it produces are added into an LDS window of 64-bit fixed-point
+// accumulators (2^-32 resolution), which makes the per-entry sums independent
+// of the order in which lanes add them.  At the end of the item the window is
+// flushed: entries owned by this item are finalised in place (theta = prior +
+// sum), entries shared with other items (query groups split over several
+// items) are merged with one 64-bit integer atomic per entry and finalised by
+// the merge kernel.  The shared (rank-indexed) tables are reduced per work
+// item into a partial buffer that the merge kernel sums.
+//
+// Expected counts that do not depend on the parameters (clicked ranks, the
+// (1,2) contributions of temporary slots the reference never writes, the
+// exact (1,1) gamma/tau1 terms of ranks before the last click) are folded
+// into per-entry constants at init, so a lane only emits the counts that
+// change between iterations.  The derivations are in the comments of each
+// model's session function.
+#ifndef MC_KERNELS_H
+#define MC_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mc {
+
+constexpr int kL = 10;                          // MAX_SERP, utils/definitions.h:50
+constexpr float kClampMax = 1.f - 0.000001f;    // param.cu:79
+constexpr float kFixScale = 4294967296.0f;      // 2^32
+constexpr double kFixInv = 1.0 / 4294967296.0;
+
+enum { kPBM = 0, kCCM = 1, kDBN = 2, kUBM = 3 };
+
+// Work item flags.
+enum { kItemInterior = 1 };
+
+struct Item {
+    int32_t s0, s1;     // session range [s0, s1) in device order
+    int32_t base;       // first qd index of the accumulator window
+    int32_t width;      // window width; 0 = no window (global atomics)
+    int32_t flags;      // kItemInterior: every window entry is owned by this item
+    int32_t pad;
+};
+
+struct TableArgs {
+    const float* val_in;      // value() of theta_t, [n_qd]
+    float* val_out;           // value() of theta_{t+1}
+    float* num_out;           // numerator of theta_{t+1}
+    const float2* base_den;   // constant (prior + fixed counts, denominator), [n_qd]
+    unsigned long long* gacc; // fixed-point accumulators of shared entries, [n_qd]
+    float* gaccf;             // float accumulators (atomic mode), [n_qd]
+};
+
+struct KArgs {
+    const int32_t* pidx;      // [kL][N] rank-major
+    const uint16_t* mask;     // [N] click bits (bit r = rank r)
+    const uint8_t* len;       // [N] ranks shown, or nullptr
+    int64_t N;
+    const Item* items;
+    int32_t n_items;
+    int32_t mode;             // 0 segmented, 1 float atomics
+    TableArgs tab[2];
+    const float* sh_val_in;   // shared table values of theta_t
+    long long* sh_partial;    // [kacc][n_items]
+    // merge kernel
+    const int32_t* shared_idx;   // entries merged across items
+    int64_t n_shared;
+    int64_t n_qd;
+    int32_t n_tables;
+    int32_t write_num;
+    // shared finalisation
+    int32_t K;                // shared table size
+    int32_t kacc;             // shared accumulators
+    const double* sh_base;    // [2*K]: base numerator, base denominator
+    const int32_t* sh_acc_map;// [2*K]: accumulator index of num/den, -1 = none
+    float* sh_val_out;
+    float* sh_num_out;
+    float* sh_den_out;
+};
+
+__device__ __forceinline__ float clamp_value(float num, float den) {   // param.cu:75-87
+    float v = num / den;
+    return v < kClampMax ? v : kClampMax;
+}
+
+__device__ __forceinline__ long long to_fix(float x) {
+    return (long long)__builtin_rintf(x * kFixScale);
+}
+
+// Highest set bit of m, or 10 when m == 0 (search.cu:160-173).
+__device__ __forceinline__ int last_click(uint32_t m) {
+    return m ? 31 - __builtin_clz(m) : kL;
+}
+
+// --------------------------------------------------------------------------
+// Emitters: where a per-qd expected count goes.
+// --------------------------------------------------------------------------
+struct EmitLds {            // fixed-point LDS window
+    unsigned long long* acc;   // [T][width]
+    int base, width;
+    __device__ __forceinline__ void add(int t, int p, float x) {
+        atomicAdd(&acc[t * width + (p - base)], (unsigned long long)to_fix(x));
+    }
+};
+struct EmitGlobalFix {      // fixed-point global (item without a window)
+    unsigned long long* g0;
+    unsigned long long* g1;
+    __device__ __forceinline__ void add(int t, int p, float x) {
+        atomicAdd(t == 0 ? &g0[p] : &g1[p], (unsigned long long)to_fix(x));
+    }
+};
+struct EmitGlobalFloat {    // float atomics (reference-like mode)
+    float* g0;
+    float* g1;
+    __device__ __forceinline__ void add(int t, int p, float x) {
+        unsafeAtomicAdd(t == 0 ? &g0[p] : &g1[p], x);
+    }
+};
+
+// --------------------------------------------------------------------------
+// Per-model session kernels.  Each takes the session index s, the rank
+// count L, the click bits mm (restricted to ranks < L) and emits the counts
+// that vary between iterations.
+// --------------------------------------------------------------------------
+
+// PBM E-step, pbm.cu:191-231.  Clicked ranks contribute (1,1) to both tables
+// (constant); unclicked ranks contribute ((a-ae)/(1-ae), 1) and ((e-ae)/(1-ae), 1).
+template <class Emit>
+__device__ __forceinline__ void pbm_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em,
+                                            const float* sh, float (&acc)[55]) {
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        if (r < L && !((mm >> r) & 1u)) {
+            int p = A.pidx[(int64_t)r * A.N + s];
+            float atr = A.tab[0].val_in[p];
+            float ex = sh[r];
+            float ae = atr * ex;
+            float na = (atr - ae) / (1 - ae);
+            float ne = (ex - ae) / (1 - ae);
+            em.add(0, p, na);
+            acc[r] += ne;
+        }
+    }
+}
+
+// Examination slot of each rank for the UBM: r(r+1)/2 + prev[r], prev = 9
+// when no earlier click (ubm.cu:201, search.cu:184-198).
+__device__ __forceinline__ void ubm_slots(uint32_t mm, int (&slot)[kL]) {
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        uint32_t before = mm & ((1u << r) - 1u);
+        int prev = before ? 31 - __builtin_clz(before) : kL - 1;
+        slot[r] = r * (r + 1) / 2 + prev;
+    }
+}
+
+// UBM E-step, ubm.cu:185-223.  The "no earlier click" slot of a rank can
+// coincide with a later rank's slot in the same session (e.g. no click
+// before rank 2 -> slot 12; click at rank 2 -> rank 4 uses slot 10 + 2 = 12);
+// the reference writes tmp[slot][session] in rank order, so only the last
+// rank writing a slot contributes to it.
+template <class Emit>
+__device__ __forceinline__ void ubm_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em,
+                                            const float* sh, float (&acc)[55]) {
+    int slot[kL];
+    ubm_slots(mm, slot);
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        if (r < L && !((mm >> r) & 1u)) {
+            bool last_writer = true;
+#pragma unroll
+            for (int q = r + 1; q < kL; q++) last_writer = last_writer && !(q < L && slot[q] == slot[r]);
+            const int row = r * (r + 1) / 2;
+            int p = A.pidx[(int64_t)r * A.N + s];
+            float atr = A.tab[0].val_in[p];
+            float ex = sh[slot[r]];
+            float ae = atr * ex;
+            float na = (atr - ae) / (1 - ae);
+            float ne = (ex - ae) / (1 - ae);
+            em.add(0, p, na);
+            ne = last_writer ? ne : 0.f;
+#pragma unroll
+            for (int j = 0; j < r; j++) acc[row + j] += (slot[r] == row + j) ? ne : 0.f;
+            acc[row + kL - 1] += (slot[r] == row + kL - 1) ? ne : 0.f;
+        }
+    }
+}
+
+// Select v[i] for a runtime i in [0, 10) without dynamic register indexing.
+__device__ __forceinline__ float pick10(const float (&v)[kL], int i) {
+    float out = v[0];
+#pragma unroll
+    for (int k = 1; k < kL; k++) out = (i == k) ? v[k] : out;
+    return out;
+}
+
+// DBN E-step, dbn.cu:207-467, for a session with at least one click.
+//
+// * compute_exm_car (dbn.cu:266-296): exam[r+1] = exam[r](t_r + g a_r (1-s_r)),
+//   t_r = g(1-a_r); car[r] = a_r + t_r car[r+1], car[L] = 0.
+// * attractiveness (dbn.cu:312-336): clicked ranks give (1,1) (constant),
+//   unclicked ranks before the last click give (0,1) (constant), unclicked
+//   ranks after it give a(1-exam)/(1-exam car).
+// * satisfaction (dbn.cu:350-378): only the last clicked rank varies:
+//   s/(1-(1-s) g car[lcr+1]); other clicked ranks give (0,1), unclicked
+//   ranks keep the (1,2) of the never-written slot (constants).
+// * gamma (dbn.cu:436-467 with the factor of factor.cu:206-274): for r < lcr
+//   every factor but phi5 is masked to 0, so phi5/sum = (phi4+phi5)/sum = 1
+//   exactly (constant (1,1)).  For r >= lcr the z=1 factors need the product
+//   of the tail click probabilities of start rank r+1, and that chain
+//   (dbn.cu:390-423) uses the parameters of ranks 0.. with the clicks of
+//   ranks r+1.. (all unclicked): its product telescopes to
+//   V_{L-1-r}, V_{k+1} = V_k - a_k G_k, G_{k+1} = G_k t_k, V_0 = G_0 = 1.
+//   Factors common to all non-zero phi (a for the clicked rank, (1-a) and
+//   the examination probability otherwise) cancel in the ratios.  The
+//   conditional examination after the last click follows
+//   E_{lcr+1} = g(1-s_lcr), E_{r+1} = E_r t_r / (1 - a_r E_r).
+template <bool VARLEN, class Emit>
+__device__ __forceinline__ void dbn_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em, float g,
+                                            float& gn, float& gd) {
+    const int lcr = last_click(mm);
+    float av[kL], sv[kL];
+    int pv[kL];
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        if (!VARLEN || r < L) {
+            pv[r] = A.pidx[(int64_t)r * A.N + s];
+            av[r] = A.tab[0].val_in[pv[r]];
+            sv[r] = A.tab[1].val_in[pv[r]];
+        } else {
+            pv[r] = 0; av[r] = 0.f; sv[r] = 0.f;
+        }
+    }
+    float t[kL], exam[kL + 1], car[kL + 1];
+    exam[0] = 1.f;
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        t[r] = g * (1 - av[r]);
+        exam[r + 1] = exam[r] * (t[r] + g * av[r] * (1 - sv[r]));
+    }
+    car[kL] = 0.f;
+#pragma unroll
+    for (int r = kL - 1; r >= 0; r--) car[r] = (!VARLEN || r < L) ? av[r] + t[r] * car[r + 1] : 0.f;
+
+    float V[kL];
+    V[0] = 1.f;
+    float G = 1.f;
+#pragma unroll
+    for (int k = 0; k < kL - 1; k++) {
+        V[k + 1] = V[k] - av[k] * G;
+        G = G * t[k];
+    }
+
+    float ex = 0.f;
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        const bool live = (!VARLEN || r < L) && r >= lcr;
+        if (live) {
+            float Tn;
+            if (VARLEN) Tn = (r + 1 < L) ? pick10(V, L - 1 - r) : 1.f;
+            else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
+            float phi4, phi5, sum;
+            if (r == lcr) {
+                float cn = (r < L - 1) ? car[r + 1] : 0.f;
+                em.add(1, pv[r], sv[r] / (1 - (1 - sv[r]) * g * cn));
+                phi4 = (1 - sv[r]) * (1 - g);
+                phi5 = (1 - sv[r]) * g * Tn;
+                float phi6 = sv[r];
+                sum = phi4 + phi5 + phi6;
+                ex = g * (1 - sv[r]);
+            } else {
+                em.add(0, pv[r], (av[r] * (1 - exam[r])) / (1 - exam[r] * car[r]));
+                float E = ex;
+                float phi0 = 1 - E;
+                phi4 = (1 - g) * E;
+                phi5 = g * Tn * E;
+                sum = phi0 + phi4 + phi5;
+                ex = ex * (t[r] / (1 - av[r] * ex));
+            }
+            gn += phi5 / sum;
+            gd += (phi4 + phi5) / sum;
+        }
+    }
+}
+
+// CCM E-step, ccm.cu:185-467, for a session with at least one click.
+//
+// * compute_exm_car (ccm.cu:245-279): t_r = (1-a_r) tau1,
+//   exam[r+1] = exam[r](t_r + a_r((1-a_r) tau2 + a_r tau3)), car as DBN.
+// * attractiveness (ccm.cu:295-326): clicked ranks give (1,2) plus, at the
+//   last click, a/(1-(tau2(1-a)+tau3 a) car[r+1]); unclicked ranks after
+//   the last click give (1-exam)a/(1-exam car).
+// * taus (ccm.cu:387-467, factor.cu:72-142): unclicked r < lcr give (1,1)
+//   to tau1 exactly (constant).  Clicked r < lcr keep phi5 and phi7 only;
+//   the tail product and examination cancel, leaving
+//   x = (1-a)tau2 / ((1-a)tau2 + a tau3) to tau2 as (x, x) and 1-x to tau3.
+//   At the last click and after it the tail product of start r+1 is
+//   V_{L-1-r} with G_{k+1} = G_k tau1 (1-a_k).  The reference forms the
+//   denominators num + phi/sum in double and narrows them (Q7).
+template <bool VARLEN, class Emit>
+__device__ __forceinline__ void ccm_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em, float t1,
+                                            float t2, float t3, float (&ta)[6]) {
+    const int lcr = last_click(mm);
+    float av[kL];
+    int pv[kL];
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        if (!VARLEN || r < L) {
+            pv[r] = A.pidx[(int64_t)r * A.N + s];
+            av[r] = A.tab[0].val_in[pv[r]];
+        } else {
+            pv[r] = 0; av[r] = 0.f;
+        }
+    }
+    float t[kL], exam[kL + 1], car[kL + 1];
+    exam[0] = 1.f;
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        t[r] = (1 - av[r]) * t1;
+        exam[r + 1] = exam[r] * (t[r] + av[r] * ((1 - av[r]) * t2 + av[r] * t3));
+    }
+    car[kL] = 0.f;
+#pragma unroll
+    for (int r = kL - 1; r >= 0; r--) car[r] = (!VARLEN || r < L) ? av[r] + t[r] * car[r + 1] : 0.f;
+
+    float V[kL];
+    V[0] = 1.f;
+    float G = 1.f;
+#pragma unroll
+    for (int k = 0; k < kL - 1; k++) {
+        V[k + 1] = V[k] - av[k] * G;
+        G = G * (t1 * (1 - av[k]));
+    }
+
+    float ex = 0.f;
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        if (VARLEN && r >= L) continue;
+        const bool clicked = (mm >> r) & 1u;
+        if (r < lcr) {
+            if (clicked) {
+                float w2 = (1 - av[r]) * t2;
+                float w3 = av[r] * t3;
+                float x = w2 / (w2 + w3);
+                float y = w3 / (w2 + w3);
+                ta[2] += x; ta[3] += x;
+                ta[4] += y; ta[5] += y;
+            }
+            continue;
+        }
+        float Tn;
+        if (VARLEN) Tn = (r + 1 < L) ? pick10(V, L - 1 - r) : 1.f;
+        else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
+        if (r == lcr) {
+            float cn = car[r + 1];
+            em.add(0, pv[r], av[r] / (1 - (t2 * (1 - av[r]) + t3 * av[r]) * cn));
+            float phi4 = (1 - av[r]) * (1 - t2);
+            float phi5 = (1 - av[r]) * t2 * Tn;
+            float phi6 = av[r] * (1 - t3);
+            float phi7 = av[r] * t3 * Tn;
+            float sum = phi4 + phi5 + phi6 + phi7;
+            float n2 = phi5 / sum;
+            float n3 = phi7 / sum;
+            ta[2] += n2;
+            ta[3] += (float)((double)n2 + (double)(phi4 / sum));
+            ta[4] += n3;
+            ta[5] += (float)((double)n3 + (double)(phi6 / sum));
+            ex = t2 * (1 - av[r]) + t3 * av[r];
+        } else {
+            em.add(0, pv[r], ((1 - exam[r]) * av[r]) / (1 - exam[r] * car[r]));
+            float E = ex;
+            float phi0 = 1 - E;
+            float phi4 = (1 - t1) * E;
+            float phi5 = t1 * Tn * E;
+            float sum = phi0 + phi4 + phi5;
+            float n1 = phi5 / sum;
+            ta[0] += n1;
+            ta[1] += (float)((double)n1 + (double)(phi4 / sum));
+            ex = ex * ((t1 * (1 - av[r])) / (1 - av[r] * ex));
+        }
+    }
+}
+
+}  // namespace mc
+
+#endif  // MC_KERNELS_H

@@ -1,0 +1,249 @@
+"""Host-side interface of the EM engine, mirroring the reference's
+ClickModel_Hst (click_models/base.cuh:30-53), Dataset (data/dataset.h) and the
+single-worker EM driver em_parallel (parallel_em/parallel_em.cu:349-746).
+
+Everything runs through libmclicks.so (the HIP engine); there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, ptr, i32p, i64p, u8p, f32p, f64p
+
+PBM, CCM, DBN, UBM = 0, 1, 2, 3          # main.cpp:178-183
+MODEL_NAMES = {PBM: "PBM", CCM: "CCM", DBN: "DBN", UBM: "UBM"}
+MAX_SERP = 10
+
+OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK = range(5)
+REDUCE_SEGMENTED, REDUCE_ATOMIC = 0, 1
+
+
+@dataclass
+class SessionSet:
+    """Sessions in the engine's host layout (session-major, 10 ranks)."""
+    query: np.ndarray      # int32 [n]
+    docs: np.ndarray       # int32 [n, 10]
+    clicks: np.ndarray     # uint8 [n, 10]
+    pidx: np.ndarray       # int32 [n, 10], -1 = pair unseen in training
+    length: np.ndarray | None = None   # uint8 [n] or None (= 10)
+
+    def __len__(self):
+        return int(self.query.shape[0])
+
+
+class Dataset:
+    """Parsed, split, sorted and indexed click log (dataset.cpp:421-455)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        L = _lib.lib()
+        n = [C.c_int64() for _ in range(4)]
+        check(L.mc_dataset_sizes(self._h, *[C.byref(x) for x in n]))
+        self.n_parsed, self.n_train, self.n_test, self.n_qd = (x.value for x in n)
+        self.train = self._export(0, self.n_train)
+        self.test = self._export(1, self.n_test)
+
+    @classmethod
+    def parse(cls, path: str, max_sessions: int = -1, test_share: float = 0.2) -> "Dataset":
+        h = C.c_void_p()
+        check(_lib.lib().mc_dataset_parse(path.encode(), int(max_sessions), float(test_share), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_sessions(cls, query, docs, clicks, test_share: float = 0.2) -> "Dataset":
+        q = np.ascontiguousarray(query, dtype=np.int32)
+        d = np.ascontiguousarray(docs, dtype=np.int32)
+        c = np.ascontiguousarray(clicks, dtype=np.uint8)
+        h = C.c_void_p()
+        check(_lib.lib().mc_dataset_from_sessions(len(q), ptr(q, i32p), ptr(d, i32p), ptr(c, u8p),
+                                                   float(test_share), C.byref(h)))
+        return cls(h)
+
+    def _export(self, which, n):
+        q = np.empty(n, np.int32)
+        d = np.empty((n, MAX_SERP), np.int32)
+        c = np.empty((n, MAX_SERP), np.uint8)
+        p = np.empty((n, MAX_SERP), np.int32)
+        if n:
+            check(_lib.lib().mc_dataset_export(self._h, which, ptr(q, i32p), ptr(d, i32p), ptr(c, u8p), ptr(p, i32p)))
+        return SessionSet(q, d, c, p)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().mc_dataset_free(self._h)
+            self._h = None
+
+
+class ClickModel:
+    """One click model on one GPU (ClickModel_Hst, base.cuh:30-53)."""
+
+    def __init__(self, model_type: int, device: int = 0, reduce_mode: int = REDUCE_SEGMENTED,
+                 item_sessions: int | None = None, item_window: int | None = None,
+                 sort_by_lcr: bool = True, block: int | None = None):
+        self.model_type = int(model_type)
+        self._h = C.c_void_p()
+        L = _lib.lib()
+        check(L.mc_model_create(self.model_type, int(device), C.byref(self._h)))
+        check(L.mc_model_set_option(self._h, OPT_REDUCE_MODE, int(reduce_mode)))
+        check(L.mc_model_set_option(self._h, OPT_SORT_BY_LCR, int(bool(sort_by_lcr))))
+        if item_sessions is not None:
+            check(L.mc_model_set_option(self._h, OPT_ITEM_SESSIONS, int(item_sessions)))
+        if item_window is not None:
+            check(L.mc_model_set_option(self._h, OPT_ITEM_WINDOW, int(item_window)))
+        if block is not None:
+            check(L.mc_model_set_option(self._h, OPT_BLOCK, int(block)))
+        self.n_qd = 0
+
+    # -- lifecycle ------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().mc_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- init_parameters (common.cu:31-61) --------------------------------------
+    def init_parameters(self, train: SessionSet, n_qd: int):
+        q = np.ascontiguousarray(train.query, np.int32)
+        p = np.ascontiguousarray(train.pidx, np.int32)
+        c = np.ascontiguousarray(train.clicks, np.uint8)
+        ln = None if train.length is None else np.ascontiguousarray(train.length, np.uint8)
+        check(_lib.lib().mc_init_parameters(self._h, len(q), ptr(q, i32p), ptr(p, i32p), ptr(c, u8p),
+                                            ptr(ln, u8p), int(n_qd)))
+        self.n_qd = int(n_qd)
+
+    def init_from_dataset(self, ds: Dataset):
+        check(_lib.lib().mc_init_from_dataset(self._h, ds._h))
+        self.n_qd = ds.n_qd
+
+    # -- EM ---------------------------------------------------------------------
+    def process_session(self):
+        check(_lib.lib().mc_process_session(self._h))
+
+    def update_parameters(self):
+        check(_lib.lib().mc_update_parameters(self._h))
+
+    def em(self, n_iter: int, use_graph: bool = True):
+        check(_lib.lib().mc_em_iterations(self._h, int(n_iter), int(bool(use_graph))))
+
+    def em_timed(self, n_iter: int):
+        t = [C.c_double() for _ in range(3)]
+        check(_lib.lib().mc_em_iterations_timed(self._h, int(n_iter), *[C.byref(x) for x in t]))
+        return {"ms_total": t[0].value, "ms_estep": t[1].value, "ms_mstep": t[2].value}
+
+    def em_graph_timed(self, n_iter: int) -> float:
+        t = C.c_double()
+        check(_lib.lib().mc_em_graph_timed(self._h, int(n_iter), C.byref(t)))
+        return t.value
+
+    def reset_parameters(self):
+        check(_lib.lib().mc_reset_parameters(self._h))
+
+    # -- tables -----------------------------------------------------------------
+    def n_tables(self) -> int:
+        n = C.c_int()
+        check(_lib.lib().mc_get_n_tables(self._h, C.byref(n)))
+        return n.value
+
+    def get_parameter_information(self):
+        """[(name, is_public, size)] per table (get_parameter_information, base.cuh:33)."""
+        out = []
+        for t in range(self.n_tables()):
+            name, pub, size = C.c_char_p(), C.c_int(), C.c_int64()
+            check(_lib.lib().mc_get_table_info(self._h, t, C.byref(name), C.byref(pub), C.byref(size)))
+            out.append((name.value.decode(), bool(pub.value), size.value))
+        return out
+
+    def get_parameters(self, t: int):
+        """(numerator, denominator) float32 arrays of table t."""
+        info = self.get_parameter_information()[t]
+        num = np.empty(info[2], np.float32)
+        den = np.empty(info[2], np.float32)
+        check(_lib.lib().mc_get_parameters(self._h, int(t), ptr(num, f32p), ptr(den, f32p)))
+        return num, den
+
+    def get_values(self, t: int):
+        """Param::value() of table t (param.cu:75-87)."""
+        num, den = self.get_parameters(t)
+        v = num / den
+        return np.where(v < np.float32(1.0) - np.float32(1e-6), v, np.float32(1.0) - np.float32(1e-6)).astype(np.float32)
+
+    def set_parameters(self, t: int, num, den):
+        num = np.ascontiguousarray(num, np.float32)
+        den = np.ascontiguousarray(den, np.float32)
+        check(_lib.lib().mc_set_parameters(self._h, int(t), ptr(num, f32p), ptr(den, f32p)))
+
+    # -- evaluation (evaluation.cpp:25-66) --------------------------------------
+    def evaluate(self, test: SessionSet, per_session: bool = False):
+        n = len(test)
+        p = np.ascontiguousarray(test.pidx, np.int32)
+        c = np.ascontiguousarray(test.clicks, np.uint8)
+        ln = None if test.length is None else np.ascontiguousarray(test.length, np.uint8)
+        logp = np.empty((n, MAX_SERP), np.float32) if per_session else None
+        full = np.empty((n, MAX_SERP), np.float32) if per_session else None
+        llh = np.empty(n, np.float32) if per_session else None
+        llh_sum = C.c_double()
+        ppl = np.empty(MAX_SERP, np.float64)
+        cnt = np.empty(MAX_SERP, np.int64)
+        check(_lib.lib().mc_evaluate(self._h, n, ptr(p, i32p), ptr(c, u8p), ptr(ln, u8p), ptr(logp, f32p),
+                                     ptr(full, f32p), ptr(llh, f32p), C.byref(llh_sum), ptr(ppl, f64p),
+                                     ptr(cnt, i64p)))
+        ppl_rank = [2.0 ** (-ppl[r] / cnt[r]) if cnt[r] else float("nan") for r in range(MAX_SERP)]
+        out = {
+            "llh_sum": llh_sum.value,
+            "log_likelihood": llh_sum.value / n,          # parallel_em.cu:704
+            "ppl_log2_sum": ppl,
+            "rank_count": cnt,
+            "perplexity_rank": ppl_rank,                  # parallel_em.cu:719-721
+            "perplexity": float(np.mean(ppl_rank)),        # parallel_em.cu:723
+        }
+        if per_session:
+            out.update(logp=logp, full=full, llh=llh)
+        return out
+
+    def get_log_conditional_click_probs(self, test: SessionSet):
+        return self.evaluate(test, per_session=True)["logp"]
+
+    def get_full_click_probs(self, test: SessionSet):
+        return self.evaluate(test, per_session=True)["full"]
+
+    # -- bookkeeping -------------------------------------------------------------
+    def get_memory_usage(self) -> int:
+        b = C.c_size_t()
+        check(_lib.lib().mc_get_memory_usage(self._h, C.byref(b)))
+        return b.value
+
+    @staticmethod
+    def compute_memory_footprint(model_type: int, n: int, n_qd: int) -> int:
+        b = C.c_size_t()
+        check(_lib.lib().mc_compute_memory_footprint(int(model_type), int(n), int(n_qd), C.byref(b)))
+        return b.value
+
+    def layout_info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(_lib.lib().mc_get_layout_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"n_items": a.value, "n_shared_entries": b.value, "max_item_sessions": c.value}
+
+
+def em_train(model_type: int, ds: Dataset, iterations: int = 50, device: int = 0, use_graph: bool = True,
+             **options):
+    """Single-worker em_parallel (parallel_em.cu:349-746): train, then evaluate
+    held-out log-likelihood and perplexity."""
+    cm = ClickModel(model_type, device=device, **options)
+    cm.init_from_dataset(ds)
+    cm.em(iterations, use_graph=use_graph)
+    ev = cm.evaluate(ds.test) if len(ds.test) else None
+    return cm, ev

@@ -1,0 +1,1015 @@
+// mc_oracle.cpp -- CPU restatement of MassiveClicks' CPU EM path (exec mode 1).
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is loaded by
+// tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+// by nothing else.  The product library (massiveclicks_amd/csrc) never links,
+// loads or calls it.
+//
+// Every function restates the reference's CPU-side code with the same
+// floating-point operation order (compile with -ffp-contract=off and without
+// -ffast-math; std::log/std::exp/std::log2 on float resolve to glibc's
+// logf/expf/log2f, as in the reference's host compile).  Citations are
+// reference-relative file:line.
+//
+// Parity pins (which test checks what) are listed per function; a function
+// whose only check is agreement with the GPU path is marked "parity unpinned".
+//
+//   * Param::value clamp ............ tests/test_oracle_math.py (closed form)
+//   * PBM/UBM E-step ................. brute-force posterior enumeration
+//   * UBM slot index (Q5) ............ explicit table + brute force over 1024 masks
+//   * DBN/CCM exam/car, atr/sat ...... brute-force enumeration of hidden states
+//   * DBN/CCM tail chain + factors ... brute force for the start-0 chain (textbook
+//                                      part); the misaligned start>0 chains (Q2) and
+//                                      the factor masking are "parity unpinned"
+//                                      beyond the line-by-line restatement
+//   * M-step ......................... invariant theta = (1,2) + sum(tmp)
+//   * parser/split/sort/qd-index ..... hand-written raw-text fixtures (tests/golden)
+//   * evaluation ..................... closed forms + brute-force full click probs
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+constexpr int kMaxSerp = 10;             // utils/definitions.h:50
+constexpr float kDefNum = 1.f;           // utils/definitions.h:42
+constexpr float kDefDen = 2.f;           // utils/definitions.h:43
+constexpr int kQueryLineLen = 15;        // utils/definitions.h:51
+constexpr int kClickLineLen = 4;         // utils/definitions.h:52
+
+enum { PBM = 0, CCM = 1, DBN = 2, UBM = 3 };   // main.cpp:178-183
+
+// ---------------------------------------------------------------------------
+// Param: (numerator, denominator) pair.  click_models/param.cuh:18-39.
+// ---------------------------------------------------------------------------
+struct Param {
+    float num = kDefNum;
+    float den = kDefDen;
+    Param() = default;
+    Param(float n, float d) : num(n), den(d) {}
+    // param.cu:75-87: num/den if below 1 - 1e-6, else 1 - 1e-6.
+    float value() const {
+        if ((num / den) < 1.f - 0.000001f) return num / den;
+        return 1.f - 0.000001f;
+    }
+    Param& operator+=(const Param& o) {   // param.cu:51-62
+        num += o.num;
+        den += o.den;
+        return *this;
+    }
+    void add(float n, float d) { num += n; den += d; }   // param.cu:146-154
+};
+
+// ---------------------------------------------------------------------------
+// Sessions.  data/search.cuh:24-55 (SERP_Hst: query + 10 x (click, doc, pidx)).
+// ---------------------------------------------------------------------------
+struct Serp {
+    int query = -1;
+    int doc[kMaxSerp];
+    int click[kMaxSerp];
+    int pidx[kMaxSerp];
+};
+
+// search.cu:160-173: last clicked rank, MAX_SERP (10) when nothing was clicked.
+inline int last_click_rank(const uint8_t* c, int len) {
+    int lcr = kMaxSerp;
+    for (int r = len - 1; r >= 0; --r) {
+        if (c[r] == 1) { lcr = r; break; }
+    }
+    return lcr;
+}
+
+// search.cu:184-198: previous clicked rank per rank, 9 when none.
+inline void prev_clicked_rank(const uint8_t* c, int len, int (&prev)[kMaxSerp]) {
+    int last = kMaxSerp - 1;
+    for (int r = 0; r < len; r++) {
+        prev[r] = last;
+        if (c[r] == 1) last = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Thread start indices for the M-step: parallel_em.cu:27-197 for one worker.
+// The E-step splits sessions evenly over the same number of threads
+// (e.g. pbm.cu:238-247); the M-step uses these query-aligned starts.
+// ---------------------------------------------------------------------------
+std::vector<int> thread_start_indices(const std::vector<int>& query, int thread_count) {
+    std::vector<int> tsi;
+    int n = (int)query.size();
+    // allocate_threads_to_devices (parallel_em.cu:133-151) with one device:
+    // every available thread goes to device 0.
+    for (int t = 0; t < thread_count; t++) tsi.push_back(-1);
+    int threads_per_device = (int)tsi.size();
+    if (n < threads_per_device) {                       // parallel_em.cu:168-171
+        tsi.assign(1, 0);
+        return tsi;
+    }
+    int stride = n / threads_per_device;                // parallel_em.cu:165
+    int unused = 0;
+    for (int tid = 0; tid < threads_per_device; ++tid) {   // parallel_em.cu:83-108
+        int curr_query = query[tid * stride];
+        int start_index = tid * stride;
+        if (tid - 1 > 0) {
+            int prev_query = query[tsi[tid - 1]];
+            if (curr_query == prev_query) {
+                bool dup = true;                        // parallel_em.cu:27-45
+                for (int i = start_index; i < n; i++) {
+                    if (query[i] != curr_query) {
+                        dup = false;
+                        start_index = i;
+                        curr_query = query[i];
+                        break;
+                    }
+                }
+                unused += dup ? 1 : 0;
+            }
+        }
+        for (int i = start_index; i >= 0; i--) {        // parallel_em.cu:56-68
+            if (query[i] != curr_query) { tsi[tid] = i + 1; break; }
+            else if (i == 0) { tsi[tid] = 0; }
+        }
+    }
+    if (unused > 0) tsi.resize(tsi.size() - unused);   // parallel_em.cu:63-72
+    return tsi;
+}
+
+// ---------------------------------------------------------------------------
+// Click models.  One object per model holds the tables and the rank-major
+// temporary buffers tmp[k * N + s] exactly as the reference's host path
+// (common.cu:31-61: tables and tmp buffers start at (1,2)).
+// ---------------------------------------------------------------------------
+struct Model {
+    int type;
+    int64_t n;                      // training sessions
+    int64_t n_qd;
+    int n_threads;
+    bool copy_dataset = false;      // DBN/CCM copy the train set into each E-step thread (dbn.cu:244)
+    bool accum_fp64 = false;        // diagnostic: M-step sums in double
+    std::vector<int> query;
+    std::vector<int32_t> pidx;      // n x 10, session-major
+    std::vector<uint8_t> click;     // n x 10
+    std::vector<uint8_t> len;       // n
+    std::vector<int> tsi;           // M-step thread starts
+
+    // tables[t], tmp[t]; t order: per-qd tables first, then the shared one(s).
+    std::vector<std::vector<Param>> tab, tmp;
+    std::vector<int> shared;        // 1 if table t is shared (public)
+    std::vector<std::string> names;
+
+    int K_shared() const { return (int)tab.back().size(); }
+};
+
+// --- generic M-step: common.cu:114-192 --------------------------------------
+void update_unique(Model& m, int t) {
+    std::vector<Param>& src = m.tmp[t];
+    std::vector<Param>& dst = m.tab[t];
+    const int64_t N = m.n;
+    auto work = [&](int64_t start, int64_t stop) {
+        for (int64_t q = start; q < stop; q++) {
+            int L = m.len[q];
+            for (int r = 0; r < L; r++) {
+                dst[m.pidx[q * kMaxSerp + r]] += src[r * N + q];   // common.cu:121
+            }
+        }
+    };
+    auto work64 = [&](int64_t start, int64_t stop, std::vector<double>& an, std::vector<double>& ad) {
+        for (int64_t q = start; q < stop; q++) {
+            int L = m.len[q];
+            for (int r = 0; r < L; r++) {
+                int p = m.pidx[q * kMaxSerp + r];
+                an[p] += src[r * N + q].num;
+                ad[p] += src[r * N + q].den;
+            }
+        }
+    };
+    if (m.accum_fp64) {
+        std::vector<double> an(dst.size(), 0.0), ad(dst.size(), 0.0);
+        work64(0, N, an, ad);
+        for (size_t p = 0; p < dst.size(); p++) {
+            dst[p].num = (float)((double)dst[p].num + an[p]);
+            dst[p].den = (float)((double)dst[p].den + ad[p]);
+        }
+        return;
+    }
+    int nt = (int)m.tsi.size();
+    if (nt <= 1) { work(0, N); return; }
+    std::vector<std::thread> th;
+    for (int tid = 0; tid < nt; tid++) {
+        int64_t a = m.tsi[tid];
+        int64_t b = tid != nt - 1 ? m.tsi[tid + 1] : N;
+        th.emplace_back(work, a, b);
+    }
+    for (auto& x : th) x.join();
+}
+
+void update_shared(Model& m, int t) {
+    std::vector<Param>& src = m.tmp[t];
+    std::vector<Param>& dst = m.tab[t];
+    const int64_t N = m.n;
+    const int K = (int)dst.size();
+    if (m.accum_fp64) {
+        for (int k = 0; k < K; k++) {
+            double sn = 0.0, sd = 0.0;
+            for (int64_t q = 0; q < N; q++) { sn += src[k * N + q].num; sd += src[k * N + q].den; }
+            dst[k].num = (float)((double)dst[k].num + sn);
+            dst[k].den = (float)((double)dst[k].den + sd);
+        }
+        return;
+    }
+    std::vector<std::mutex> mtx(K);
+    auto work = [&](int tid, int64_t start, int64_t stop) {   // common.cu:154-174
+        std::vector<Param> local(K, Param(0.f, 0.f));
+        for (int64_t q = start; q < stop; q++)
+            for (int k = 0; k < K; k++) local[k] += src[k * N + q];
+        int offset = tid % K;
+        for (int k = 0; k < K; k++) {
+            int idx = (k + offset) % K;
+            std::lock_guard<std::mutex> g(mtx[idx]);
+            dst[idx] += local[idx];
+        }
+    };
+    int nt = (int)m.tsi.size();
+    if (nt <= 1) { work(0, 0, N); return; }
+    std::vector<std::thread> th;
+    for (int tid = 0; tid < nt; tid++) {
+        int64_t a = m.tsi[tid];
+        int64_t b = tid != nt - 1 ? m.tsi[tid + 1] : N;
+        th.emplace_back(work, tid, a, b);
+    }
+    for (auto& x : th) x.join();
+}
+
+// --- PBM E-step: pbm.cu:191-231 ---------------------------------------------
+void pbm_session(Model& m, int64_t q) {
+    const int64_t N = m.n;
+    const int32_t* p = &m.pidx[q * kMaxSerp];
+    const uint8_t* c = &m.click[q * kMaxSerp];
+    int L = m.len[q];
+    for (int r = 0; r < L; r++) {
+        float atr = m.tab[0][p[r]].value();
+        float ex = m.tab[1][r].value();
+        float na = 1, ne = 1;
+        if (c[r] == 0) {
+            float atr_ex = atr * ex;
+            na = (atr - atr_ex) / (1 - atr_ex);
+            ne = (ex - atr_ex) / (1 - atr_ex);
+        }
+        m.tmp[0][r * N + q] = Param(na, 1);
+        m.tmp[1][r * N + q] = Param(ne, 1);
+    }
+}
+
+// --- UBM E-step: ubm.cu:185-223 (slot index r(r+1)/2 + prev[r], Q5) ---------
+void ubm_session(Model& m, int64_t q) {
+    const int64_t N = m.n;
+    const int32_t* p = &m.pidx[q * kMaxSerp];
+    const uint8_t* c = &m.click[q * kMaxSerp];
+    int L = m.len[q];
+    int prev[kMaxSerp] = {0};
+    prev_clicked_rank(c, L, prev);
+    for (int r = 0; r < L; r++) {
+        int slot = r * (r + 1) / 2 + prev[r];
+        float atr = m.tab[0][p[r]].value();
+        float ex = m.tab[1][slot].value();
+        float na = 1, ne = 1;
+        if (c[r] == 0) {
+            float atr_ex = atr * ex;
+            na = (atr - atr_ex) / (1 - atr_ex);
+            ne = (ex - atr_ex) / (1 - atr_ex);
+        }
+        m.tmp[0][r * N + q] = Param(na, 1);
+        m.tmp[1][slot * N + q] = Param(ne, 1);
+    }
+}
+
+// --- DBN factor, CPU branch: factor.cu:206-274 ------------------------------
+struct DbnFactor {
+    const float (*cp)[kMaxSerp];
+    const float* exam_probs;
+    int click, lcr, rank, L;
+    float attr, sat, gamma;
+    float compute(int x, int y, int z) const {
+        float lp = 0.f;
+        if (click == 0) {
+            if (y == 1) return 0.f;
+            lp += std::log(1 - attr);
+            if (x == 1) {
+                if (z == 1) lp += std::log(gamma);
+                else lp += std::log(1 - gamma);
+            } else if (z == 1) {
+                return 0.f;
+            }
+        } else {
+            if (x == 0) return 0.f;
+            lp += std::log(attr);
+            if (y == 0) {
+                lp += std::log(1 - sat);
+                if (z == 1) lp += std::log(gamma);
+                else lp += std::log(1 - gamma);
+            } else {
+                if (z == 1) return 0.f;
+                lp += std::log(sat);
+            }
+        }
+        if (z == 0) {
+            if (lcr >= rank + 1) return 0.f;
+        } else if (rank + 1 < L) {
+            for (int j = 0; j < L - rank - 1; j++) lp += std::log(cp[rank + 1][j]);
+        }
+        float ev = exam_probs[rank];
+        if (x == 1) lp += std::log(ev);
+        else lp += std::log(1 - ev);
+        return std::exp(lp);
+    }
+};
+
+// --- CCM factor, CPU branch: factor.cu:72-142 -------------------------------
+struct CcmFactor {
+    const float (*cp)[kMaxSerp];
+    const float* exam_probs;
+    int click, lcr, rank, L;
+    float attr, tau1, tau2, tau3;
+    float compute(int x, int y, int z) const {
+        float lp = 0.f;
+        if (click == 0) {
+            if (y == 1) return 0.f;
+            lp += std::log(1 - attr);
+            if (x == 1) {
+                if (z == 1) lp += std::log(tau1);
+                else lp += std::log(1 - tau1);
+            } else if (z == 1) {
+                return 0.f;
+            }
+        } else {
+            if (x == 0) return 0.f;
+            lp += std::log(attr);
+            if (y == 0) {
+                lp += std::log(1 - attr);
+                if (z == 1) lp += std::log(tau2);
+                else lp += std::log(1 - tau2);
+            } else {
+                lp += std::log(attr);
+                if (z == 1) lp += std::log(tau3);
+                else lp += std::log(1 - tau3);
+            }
+        }
+        if (z == 0) {
+            if (lcr >= rank + 1) return 0.f;
+        } else if (rank + 1 < L) {
+            for (int j = 0; j < L - rank - 1; j++) lp += std::log(cp[rank + 1][j]);
+        }
+        if (x == 1) lp += std::log(exam_probs[rank]);
+        else lp += std::log(1 - exam_probs[rank]);
+        return std::exp(lp);
+    }
+};
+
+// factor_inputs (dbn.cuh:75-82, ccm.cuh:72-79): index = 4x + 2y + z.
+constexpr int kFactorInputs[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 1},
+                                     {1, 0, 0}, {1, 0, 1}, {1, 1, 0}, {1, 1, 1}};
+
+// --- DBN E-step: dbn.cu:207-467 ---------------------------------------------
+void dbn_session(Model& m, int64_t q) {
+    const int64_t N = m.n;
+    const int32_t* p = &m.pidx[q * kMaxSerp];
+    const uint8_t* c = &m.click[q * kMaxSerp];
+    const int L = m.len[q];
+    std::vector<Param>& A = m.tab[0];
+    std::vector<Param>& S = m.tab[1];
+    std::vector<Param>& G = m.tab[2];
+
+    int lcr = last_click_rank(c, L);
+    float click_probs[kMaxSerp][kMaxSerp] = {{0.f}};
+    float exam_probs[kMaxSerp + 1];
+    float exam[kMaxSerp + 1];
+    float car[kMaxSerp + 1] = {0.f};
+
+    m.tmp[2][q] = Param(0.f, 0.f);                                  // dbn.cu:222
+
+    // compute_exm_car: dbn.cu:266-296
+    exam[0] = 1.f;
+    float helper[kMaxSerp][2];
+    for (int r = 0; r < L;) {
+        float a = A[p[r]].value();
+        float s = S[p[r]].value();
+        float g = G[0].value();
+        float ex = exam[r];
+        float temp = g * (1 - a);
+        ex *= temp + g * a * (1 - s);
+        helper[r][0] = a;
+        helper[r][1] = temp;
+        r += 1;
+        exam[r] = ex;
+    }
+    car[L] = 0.f;
+    for (int r = L - 1; r > -1; r--) car[r] = helper[r][0] + helper[r][1] * car[r + 1];
+
+    // compute_dbn_atr: dbn.cu:312-336
+    for (int r = 0; r < L; r++) {
+        float nu = 0.f, de = 1.f;
+        if (c[r] == 1) {
+            nu += 1.f;
+        } else if (r >= lcr) {
+            float a = A[p[r]].value();
+            float ev = exam[r];
+            float cv = car[r];
+            nu += (a * (1 - ev)) / (1 - ev * cv);
+        }
+        m.tmp[0][r * N + q] = Param(nu, de);
+    }
+
+    // compute_dbn_sat: dbn.cu:350-378 (written only at clicked ranks, Q3)
+    for (int r = 0; r < L; r++) {
+        if (c[r] == 1) {
+            float nu = 0.f, de = 1.f;
+            if (r == lcr) {
+                float s = S[p[r]].value();
+                float g = G[0].value();
+                float cv = r < L - 1 ? car[r + 1] : 0.f;
+                nu += s / (1 - (1 - s) * g * cv);
+            }
+            m.tmp[1][r * N + q] = Param(nu, de);
+        }
+    }
+
+    // get_tail_clicks: dbn.cu:390-423 (params of rank j, click of rank start+j: Q2)
+    exam_probs[0] = 1.f;
+    for (int start = 0; start < L; start++) {
+        float ex = 1.f;
+        int j = 0;
+        for (int res = start; res < L; res++) {
+            float a = A[p[j]].value();
+            float s = S[p[j]].value();
+            float g = G[0].value();
+            float cp;
+            if (c[res] == 1) {
+                cp = a * ex;
+                ex = g * (1 - s);
+            } else {
+                cp = 1 - a * ex;
+                ex *= g * (1 - a) / cp;
+            }
+            click_probs[start][j] = cp;
+            if (start == 0) exam_probs[j + 1] = ex;
+            j++;
+        }
+    }
+
+    // compute_gamma: dbn.cu:436-467
+    for (int r = 0; r < L; r++) {
+        DbnFactor f{click_probs, exam_probs, c[r], lcr, r, L,
+                    A[p[r]].value(), S[p[r]].value(), G[0].value()};
+        float fv[8];
+        float sum = 0.f;
+        for (int i = 0; i < 8; i++) {
+            float v = f.compute(kFactorInputs[i][0], kFactorInputs[i][1], kFactorInputs[i][2]);
+            fv[i] = v;
+            sum += v;
+        }
+        float nu = fv[5] / sum;
+        float de = (fv[4] + fv[5]) / sum;
+        m.tmp[2][q].add(nu, de);
+    }
+}
+
+// --- CCM E-step: ccm.cu:185-467 ---------------------------------------------
+void ccm_session(Model& m, int64_t q) {
+    const int64_t N = m.n;
+    const int32_t* p = &m.pidx[q * kMaxSerp];
+    const uint8_t* c = &m.click[q * kMaxSerp];
+    const int L = m.len[q];
+    std::vector<Param>& A = m.tab[0];
+    std::vector<Param>& T = m.tab[1];
+
+    int lcr = last_click_rank(c, L);
+    float click_probs[kMaxSerp][kMaxSerp] = {{0.f}};
+    float exam_probs[kMaxSerp + 1];
+    float exam[kMaxSerp + 1];
+    float car[kMaxSerp + 1] = {0.f};
+
+    m.tmp[1][q] = Param(0.f, 0.f);                                   // ccm.cu:201-203
+    m.tmp[1][N + q] = Param(0.f, 0.f);
+    m.tmp[1][2 * N + q] = Param(0.f, 0.f);
+
+    // compute_exm_car: ccm.cu:245-279
+    exam[0] = 1.f;
+    float helper[kMaxSerp][2];
+    for (int r = 0; r < L;) {
+        float a = A[p[r]].value();
+        float t1 = T[0].value(), t2 = T[1].value(), t3 = T[2].value();
+        float ex = exam[r];
+        float temp = (1 - a) * t1;
+        ex *= temp + a * ((1 - a) * t2 + a * t3);
+        helper[r][0] = a;
+        helper[r][1] = temp;
+        r += 1;
+        exam[r] = ex;
+    }
+    car[L] = 0.f;
+    for (int r = L - 1; r > -1; r--) car[r] = helper[r][0] + helper[r][1] * car[r + 1];
+
+    // compute_ccm_atr: ccm.cu:295-326
+    for (int r = 0; r < L; r++) {
+        float nu = 0.f, de = 1.f;
+        float a = A[p[r]].value();
+        float ev = exam[r];
+        if (c[r] == 1) {
+            nu += 1;
+            de += 1;
+        } else if (r >= lcr) {
+            float cv = car[r];
+            nu += ((1 - ev) * a) / (1 - ev * cv);
+        }
+        if (c[r] == 1 && r == lcr) {
+            float cv = car[r + 1];
+            nu += a / (1 - (T[1].value() * (1 - a) + T[2].value() * a) * cv);
+        }
+        m.tmp[0][r * N + q] = Param(nu, de);
+    }
+
+    // get_tail_clicks: ccm.cu:338-374 (Q2)
+    exam_probs[0] = 1.f;
+    for (int start = 0; start < L; start++) {
+        float ex = 1.f;
+        int j = 0;
+        for (int res = start; res < L; res++) {
+            float a = A[p[j]].value();
+            float t1 = T[0].value(), t2 = T[1].value(), t3 = T[2].value();
+            float cp;
+            if (c[res] == 1) {
+                cp = a * ex;
+                ex = t2 * (1 - a) + t3 * a;
+            } else {
+                cp = 1 - a * ex;
+                ex *= t1 * (1 - a) / cp;
+            }
+            click_probs[start][j] = cp;
+            if (start == 0) exam_probs[j + 1] = ex;
+            j++;
+        }
+    }
+
+    // compute_taus: ccm.cu:387-467 (double temporaries narrowed on add, Q7)
+    for (int r = 0; r < L; r++) {
+        CcmFactor f{click_probs, exam_probs, c[r], lcr, r, L,
+                    A[p[r]].value(), T[0].value(), T[1].value(), T[2].value()};
+        float fv[8];
+        float sum = 0.f;
+        for (int i = 0; i < 8; i++) {
+            float v = f.compute(kFactorInputs[i][0], kFactorInputs[i][1], kFactorInputs[i][2]);
+            fv[i] = v;
+            sum += v;
+        }
+        if (c[r] == 0) {
+            double nu{(fv[5] + fv[7]) / sum};
+            double de{nu + ((fv[4] + fv[6]) / sum)};
+            m.tmp[1][q].add((float)nu, (float)de);
+        } else {
+            double nu2{fv[5] / sum};
+            double de2{nu2 + ((fv[4]) / sum)};
+            m.tmp[1][N + q].add((float)nu2, (float)de2);
+            double nu3{fv[7] / sum};
+            double de3{nu3 + ((fv[6]) / sum)};
+            m.tmp[1][2 * N + q].add((float)nu3, (float)de3);
+        }
+    }
+}
+
+void estep(Model& m) {
+    auto body = [](Model* mm, int64_t a, int64_t b) {
+        for (int64_t q = a; q < b; q++) {
+            switch (mm->type) {
+                case PBM: pbm_session(*mm, q); break;
+                case UBM: ubm_session(*mm, q); break;
+                case DBN: dbn_session(*mm, q); break;
+                case CCM: ccm_session(*mm, q); break;
+            }
+        }
+    };
+    int nt = (int)m.tsi.size();
+    if (nt <= 1) { body(&m, 0, m.n); return; }
+    // Even split over the threads (pbm.cu:238-247).
+    int64_t part = m.n / nt, left = m.n % nt, start = 0, stop = 0;
+    const bool copy = m.copy_dataset && (m.type == DBN || m.type == CCM);
+    std::vector<std::thread> th;
+    for (int tid = 0; tid < nt; tid++) {
+        stop += tid < left ? part + 1 : part;
+        if (copy) {
+            // dbn.cu:244 / ccm.cu:223 hand the train set (124-byte SERP_Hst
+            // per session) to std::thread by value: one serial copy per thread
+            // on the calling thread.  Replayed for timing fidelity only.
+            std::vector<char> blob((size_t)m.n * 124, 1);
+            th.emplace_back([body, &m, start, stop](std::vector<char> b) { body(&m, start, stop); (void)b; },
+                            std::move(blob));
+        } else {
+            th.emplace_back(body, &m, start, stop);
+        }
+        start += tid < left ? part + 1 : part;
+    }
+    for (auto& x : th) x.join();
+}
+
+void reset_tables(Model& m) {                 // common.cu:71-82
+    for (auto& t : m.tab) std::fill(t.begin(), t.end(), Param(kDefNum, kDefDen));
+}
+
+void mstep(Model& m) {
+    reset_tables(m);
+    // Update order per model: pbm.cu:179-182, ubm.cu:173-176, dbn.cu:194-198, ccm.cu:173-176.
+    for (size_t t = 0; t < m.tab.size(); t++) {
+        if (m.shared[t]) update_shared(m, (int)t);
+        else update_unique(m, (int)t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Evaluation (per test session).  pbm.cu:376-439, ubm.cu:373-468,
+// dbn.cu:611-683, ccm.cu:604-671; unseen qd pairs (pidx -1) use 1/2.
+// ---------------------------------------------------------------------------
+inline float pval(const std::vector<Param>& t, int p) {
+    float v{(float)kDefNum / (float)kDefDen};
+    if (p != -1) v = t[p].value();
+    return v;
+}
+
+void eval_session(const Model& m, const int32_t* p, const uint8_t* c, int L, float* logp, float* full) {
+    switch (m.type) {
+        case PBM: {
+            for (int r = 0; r < L; r++) {
+                float atr = pval(m.tab[0], p[r]);
+                float ex = m.tab[1][r].value();
+                float ae = atr * ex;
+                logp[r] = c[r] == 1 ? std::log(ae) : std::log(1 - ae);
+                full[r] = c[r] == 1 ? ae : 1 - ae;
+            }
+            break;
+        }
+        case UBM: {
+            int prev[kMaxSerp] = {0};
+            prev_clicked_rank(c, L, prev);
+            for (int r = 0; r < L; r++) {
+                float atr = pval(m.tab[0], p[r]);
+                float ex = m.tab[1][r * (r + 1) / 2 + prev[r]].value();
+                float ae = atr * ex;
+                logp[r] = c[r] == 1 ? std::log(ae) : std::log(1 - ae);
+            }
+            // ubm.cu:412-468: marginal over the previous click rank.
+            float tmp_full[kMaxSerp];
+            for (int r = 0; r < L; r++) {
+                float click_prob{0};
+                for (int rp = -1; rp < r; rp++) {
+                    float no_click_between = 1.f;
+                    int crp = rp == -1 ? 9 : rp;
+                    for (int rb = rp + 1; rb < r; rb++) {
+                        float av = pval(m.tab[0], p[rb]);
+                        float ev = m.tab[1][rb * (rb + 1) / 2 + crp].value();
+                        no_click_between *= 1 - av * ev;
+                    }
+                    float av = pval(m.tab[0], p[r]);
+                    float ev = m.tab[1][r * (r + 1) / 2 + crp].value();
+                    float temp{no_click_between * (av * ev)};
+                    if (rp >= 0) click_prob += tmp_full[rp] * temp;
+                    else click_prob += temp;
+                }
+                tmp_full[r] = click_prob;
+                full[r] = c[r] == 1 ? click_prob : 1 - click_prob;
+            }
+            break;
+        }
+        case DBN: {
+            float g = m.tab[2][0].value();
+            float ex{1.f};
+            for (int r = 0; r < L; r++) {
+                float a = pval(m.tab[0], p[r]);
+                float s = pval(m.tab[1], p[r]);
+                float cp;
+                if (c[r] == 1) {
+                    cp = a * ex;
+                    ex = g * (1 - s);
+                } else {
+                    cp = 1 - a * ex;
+                    ex *= g * (1 - a) / cp;
+                }
+                logp[r] = std::log(cp);
+            }
+            ex = 1.f;
+            for (int r = 0; r < L; r++) {
+                float a = pval(m.tab[0], p[r]);
+                float s = pval(m.tab[1], p[r]);
+                float ae = a * ex;
+                full[r] = c[r] == 1 ? ae : 1 - ae;
+                ex *= g * (1 - a) + g * a * (1 - s);
+            }
+            break;
+        }
+        case CCM: {
+            float t1 = m.tab[1][0].value(), t2 = m.tab[1][1].value(), t3 = m.tab[1][2].value();
+            float ex{1.f};
+            for (int r = 0; r < L; r++) {
+                float a = pval(m.tab[0], p[r]);
+                float cp;
+                if (c[r] == 1) {
+                    cp = a * ex;
+                    ex = t2 * (1 - a) + t3 * a;
+                } else {
+                    cp = 1 - a * ex;
+                    ex *= t1 * (1 - a) / cp;
+                }
+                logp[r] = std::log(cp);
+            }
+            ex = 1.f;
+            for (int r = 0; r < L; r++) {
+                float a = pval(m.tab[0], p[r]);
+                float ae = a * ex;
+                full[r] = c[r] == 1 ? ae : 1 - ae;
+                ex *= (1 - a) * t1 + a * ((1 - a) * t2 + a * t3);
+            }
+            break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dataset: parse (dataset.cpp:224-305, search.cu:93-122), split
+// (dataset.cpp:385-396), sort (dataset.cpp:17-21), qd indices
+// (dataset.cpp:314-374).
+// ---------------------------------------------------------------------------
+struct Dataset {
+    int64_t n_parsed = 0;
+    std::vector<Serp> train, test;
+    int64_t n_qd = 0;
+};
+
+void prepare(Dataset& d, std::vector<Serp>& all, float test_share) {
+    d.n_parsed = (int64_t)all.size();
+    int n_queries = (int)all.size();
+    int n_train = static_cast<int>(n_queries * (1 - test_share) - 0.5) + 1;   // dataset.cpp:390
+    d.train.assign(all.begin(), all.begin() + n_train);
+    d.test.assign(all.begin() + n_train, all.end());
+    std::sort(d.train.begin(), d.train.end(),
+              [](const Serp& a, const Serp& b) { return a.query < b.query; });   // dataset.cpp:19-20
+    // assign_train_parameter_indices: dataset.cpp:314-339
+    std::unordered_map<int, std::unordered_map<int, int>> qd;
+    size_t param_index = 0;
+    for (Serp& s : d.train) {
+        auto qit = qd.emplace(s.query, std::unordered_map<int, int>{}).first;
+        for (int r = 0; r < kMaxSerp; r++) {
+            auto ins = qit->second.emplace(s.doc[r], (int)param_index);
+            s.pidx[r] = ins.first->second;
+            if (ins.second) param_index++;
+        }
+    }
+    d.n_qd = (int64_t)param_index;
+    // assign_test_parameter_indices: dataset.cpp:349-374 (remove_if keeps order)
+    d.test.erase(std::remove_if(d.test.begin(), d.test.end(), [&](Serp& s) {
+        auto qit = qd.find(s.query);
+        if (qit == qd.end()) return true;
+        bool found = false;
+        for (int r = 0; r < kMaxSerp; r++) {
+            auto dit = qit->second.find(s.doc[r]);
+            if (dit != qit->second.end()) {
+                found = true;
+                s.pidx[r] = dit->second;
+            }
+        }
+        return !found;
+    }), d.test.end());
+}
+
+}  // namespace
+
+// ===========================================================================
+// C interface (ctypes).
+// ===========================================================================
+extern "C" {
+
+struct orc_dataset { Dataset d; };
+struct orc_model { Model m; };
+
+// Parse a raw click log.  Returns NULL and fills err on failure.
+// Deviations from undefined behaviour in the reference (documented): an empty
+// line is skipped without touching the session-id state (the reference indexes
+// an empty token vector), and a line seen before the first query line never
+// causes an uninitialised SERP to be emitted.
+orc_dataset* orc_parse_raw(const char* path, int max_sessions, float test_share, char* err, int errlen) {
+    std::ifstream f(path);
+    if (!f.is_open()) {                                  // dataset.cpp:226-229
+        std::snprintf(err, errlen, "cannot open %s", path);
+        return nullptr;
+    }
+    std::vector<Serp> all;
+    std::string line, element;
+    Serp cur;
+    bool have_cur = false;
+    int n_queries = 0, sid_curr = 0, sid_prev = -1;
+    try {
+        while (std::getline(f, line) && (n_queries < max_sessions || max_sessions == -1)) {
+            std::istringstream ssi(line);
+            std::vector<std::string> v;
+            while (ssi >> element) v.push_back(element);
+            if (v.empty()) continue;
+            try {
+                sid_curr = std::stoi(v[0]);
+            } catch (std::invalid_argument&) {
+                continue;                                 // dataset.cpp:258-263
+            }
+            if ((int)v.size() == kQueryLineLen) {         // dataset.cpp:268-276
+                if (sid_prev != -1 && have_cur) {
+                    all.push_back(cur);
+                    n_queries++;
+                }
+                cur = Serp();
+                cur.query = std::stoi(v[3]);              // search.cu:93-99
+                for (int i = 0; i < kMaxSerp; i++) {
+                    cur.doc[i] = std::stoi(v[i + 5]);
+                    cur.click[i] = 0;
+                    cur.pidx[i] = -1;
+                }
+                have_cur = true;
+            } else if ((int)v.size() == kClickLineLen) {  // dataset.cpp:279-281
+                if (have_cur) {
+                    int doc = std::stoi(v[3]);            // search.cu:111-122
+                    for (int j = 0; j < kMaxSerp; j++) {
+                        if (cur.doc[j] == doc) { cur.click[j] = 1; break; }
+                    }
+                }
+            }
+            sid_prev = sid_curr;                          // dataset.cpp:288
+        }
+    } catch (std::exception& e) {
+        std::snprintf(err, errlen, "malformed token: %s", e.what());
+        return nullptr;
+    }
+    if (all.empty()) {
+        std::snprintf(err, errlen, "no complete session in %s", path);
+        return nullptr;
+    }
+    int n = (int)all.size();
+    int n_train = static_cast<int>(n * (1 - test_share) - 0.5) + 1;
+    if (n_train < 1 || n_train > n) {
+        std::snprintf(err, errlen, "test share %g leaves an invalid train split", (double)test_share);
+        return nullptr;
+    }
+    orc_dataset* h = new orc_dataset;
+    prepare(h->d, all, test_share);
+    return h;
+}
+
+// Same preparation from sessions already in memory (docs/clicks n x 10).
+orc_dataset* orc_from_sessions(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
+                               float test_share) {
+    std::vector<Serp> all((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        all[i].query = query[i];
+        for (int r = 0; r < kMaxSerp; r++) {
+            all[i].doc[r] = docs[i * kMaxSerp + r];
+            all[i].click[r] = clicks[i * kMaxSerp + r];
+            all[i].pidx[r] = -1;
+        }
+    }
+    int n_train = static_cast<int>((int)n * (1 - test_share) - 0.5) + 1;
+    if (n < 1 || n_train < 1 || n_train > n) return nullptr;
+    orc_dataset* h = new orc_dataset;
+    prepare(h->d, all, test_share);
+    return h;
+}
+
+void orc_dataset_sizes(const orc_dataset* h, int64_t* n_parsed, int64_t* n_train, int64_t* n_test, int64_t* n_qd) {
+    *n_parsed = h->d.n_parsed;
+    *n_train = (int64_t)h->d.train.size();
+    *n_test = (int64_t)h->d.test.size();
+    *n_qd = h->d.n_qd;
+}
+
+void orc_dataset_export(const orc_dataset* h, int which, int32_t* query, int32_t* docs, uint8_t* clicks, int32_t* pidx) {
+    const std::vector<Serp>& v = which == 0 ? h->d.train : h->d.test;
+    for (size_t i = 0; i < v.size(); i++) {
+        query[i] = v[i].query;
+        for (int r = 0; r < kMaxSerp; r++) {
+            docs[i * kMaxSerp + r] = v[i].doc[r];
+            clicks[i * kMaxSerp + r] = (uint8_t)v[i].click[r];
+            pidx[i * kMaxSerp + r] = v[i].pidx[r];
+        }
+    }
+}
+
+void orc_dataset_free(orc_dataset* h) { delete h; }
+
+// Model creation: tables and tmp buffers at (1,2) (common.cu:31-61).  len may
+// be NULL (every session has MAX_SERP ranks).  n_threads >= 1.
+orc_model* orc_model_create(int type, int64_t n, const int32_t* query, const int32_t* pidx, const uint8_t* clicks,
+                            const uint8_t* len, int64_t n_qd, int n_threads) {
+    if (type < 0 || type > 3 || n < 1 || n_qd < 1 || n_threads < 1) return nullptr;
+    orc_model* h = new orc_model;
+    Model& m = h->m;
+    m.type = type;
+    m.n = n;
+    m.n_qd = n_qd;
+    m.n_threads = n_threads;
+    m.query.assign(query, query + n);
+    m.pidx.assign(pidx, pidx + n * kMaxSerp);
+    m.click.assign(clicks, clicks + n * kMaxSerp);
+    m.len.assign((size_t)n, (uint8_t)kMaxSerp);
+    if (len) m.len.assign(len, len + n);
+    m.tsi = thread_start_indices(m.query, n_threads);
+    auto add = [&](const char* name, int64_t size, int64_t tmp_size, int shared) {
+        m.names.push_back(name);
+        m.tab.emplace_back((size_t)size, Param(kDefNum, kDefDen));
+        m.tmp.emplace_back((size_t)tmp_size, Param());
+        m.shared.push_back(shared);
+    };
+    switch (type) {   // table sizes: pbm.cu:80-96, ubm.cu:74-90, dbn.cu:76-105, ccm.cu:74-90
+        case PBM: add("attractiveness", n_qd, n * kMaxSerp, 0); add("examination", kMaxSerp, n * kMaxSerp, 1); break;
+        case UBM: add("attractiveness", n_qd, n * kMaxSerp, 0); add("examination", 55, n * 55, 1); break;
+        case CCM: add("attractiveness", n_qd, n * kMaxSerp, 0); add("tau", 3, n * 3, 1); break;
+        case DBN: add("attractiveness", n_qd, n * kMaxSerp, 0); add("satisfaction", n_qd, n * kMaxSerp, 0);
+                  add("gamma", 1, n, 1); break;
+    }
+    return h;
+}
+
+void orc_model_set_options(orc_model* h, int accum_fp64, int copy_dataset) {
+    h->m.accum_fp64 = accum_fp64 != 0;
+    h->m.copy_dataset = copy_dataset != 0;
+}
+int orc_model_threads(const orc_model* h) { return (int)h->m.tsi.size(); }
+void orc_estep(orc_model* h) { estep(h->m); }
+void orc_mstep(orc_model* h) { mstep(h->m); }
+void orc_iterate(orc_model* h, int n_iter) {    // parallel_em.cu:461-609 (sync is a no-op for one worker)
+    for (int i = 0; i < n_iter; i++) { estep(h->m); mstep(h->m); }
+}
+int orc_n_tables(const orc_model* h) { return (int)h->m.tab.size(); }
+int64_t orc_table_size(const orc_model* h, int t) { return (int64_t)h->m.tab[t].size(); }
+int64_t orc_tmp_size(const orc_model* h, int t) { return (int64_t)h->m.tmp[t].size(); }
+const char* orc_table_name(const orc_model* h, int t) { return h->m.names[t].c_str(); }
+void orc_get_table(const orc_model* h, int t, float* num, float* den, float* val) {
+    const auto& v = h->m.tab[t];
+    for (size_t i = 0; i < v.size(); i++) {
+        num[i] = v[i].num;
+        den[i] = v[i].den;
+        if (val) val[i] = v[i].value();
+    }
+}
+void orc_set_table(orc_model* h, int t, const float* num, const float* den) {
+    auto& v = h->m.tab[t];
+    for (size_t i = 0; i < v.size(); i++) v[i] = Param(num[i], den[i]);
+}
+void orc_get_tmp(const orc_model* h, int t, float* num, float* den) {
+    const auto& v = h->m.tmp[t];
+    for (size_t i = 0; i < v.size(); i++) { num[i] = v[i].num; den[i] = v[i].den; }
+}
+
+// Held-out evaluation (evaluation.cpp:25-66, parallel_em.cu:694-724).
+// logp/full (n x 10) and llh (n) are optional.  Sessions shorter than
+// MAX_SERP average over their own length (extension; L = 10 is the reference).
+void orc_eval(const orc_model* h, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+              float* logp, float* full, float* llh, float* llh_total_f32, double* llh_total_f64,
+              float* ppl_rank_f32, double* ppl_rank_f64, int64_t* rank_count) {
+    float tot32 = 0.f;
+    double tot64 = 0.0;
+    float pr32[kMaxSerp] = {0};
+    double pr64[kMaxSerp] = {0};
+    int64_t cnt[kMaxSerp] = {0};
+    for (int64_t i = 0; i < n; i++) {
+        int L = len ? len[i] : kMaxSerp;
+        float lp[kMaxSerp], fp[kMaxSerp];
+        eval_session(h->m, pidx + i * kMaxSerp, clicks + i * kMaxSerp, L, lp, fp);
+        float acc = 0.f;
+        for (int r = 0; r < L; r++) acc = acc + lp[r];       // std::accumulate(..., 0.f)
+        float s = acc / L;                                     // evaluation.cpp:36 (L = MAX_SERP)
+        if (llh) llh[i] = s;
+        tot32 += s;
+        tot64 += (double)s;
+        for (int r = 0; r < L; r++) {
+            float l2 = std::log2(fp[r]);
+            pr32[r] += l2;
+            pr64[r] += (double)l2;
+            cnt[r]++;
+        }
+        for (int r = 0; r < kMaxSerp; r++) {
+            if (logp) logp[i * kMaxSerp + r] = r < L ? lp[r] : 0.f;
+            if (full) full[i * kMaxSerp + r] = r < L ? fp[r] : 0.f;
+        }
+    }
+    *llh_total_f32 = tot32;
+    *llh_total_f64 = tot64;
+    for (int r = 0; r < kMaxSerp; r++) {
+        ppl_rank_f32[r] = pr32[r];
+        ppl_rank_f64[r] = pr64[r];
+        rank_count[r] = cnt[r];
+    }
+}
+
+void orc_model_free(orc_model* h) { delete h; }
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+"""GPU parity of the EM hot path against the CPU oracle (through the C ABI).
+
+Bar (stated per test): FP32 parameters within 1e-5 absolute after 50
+iterations; held-out log-likelihood and per-rank perplexity sums within 1e-6
+relative of the FP64 total of the oracle's per-session FP32 terms.
+
+The reference sums the M-step in FP32 in session order; at tens of thousands
+of sessions that sum is itself off by more than 1e-5 for the rank-indexed
+tables (its rounding error grows with the count).  The engine's sums are
+exact (64-bit fixed point) and rounded once.  So the parameters are gated
+against the oracle with FP64 M-step sums (same E-step arithmetic) at the
+larger sizes, and against the reference-faithful FP32 oracle at a size where
+that oracle's own error is below the bar.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from massiveclicks_amd import CCM, DBN, PBM, UBM, ClickModel, REDUCE_ATOMIC, REDUCE_SEGMENTED, synth
+from massiveclicks_amd.model import SessionSet
+
+pytestmark = pytest.mark.gpu
+
+TOL_PARAM = 1e-5
+TOL_LL = 1e-6
+ITERS = 50
+
+CASES = {
+    PBM: ("pbm", 20000, 1500),
+    UBM: ("ubm", 20000, 1500),
+    DBN: ("dbn", 6000, 600),
+    CCM: ("ccm", 6000, 600),
+}
+
+SMALL = {PBM: 2500, UBM: 2500, DBN: 2500, CCM: 2500}
+
+_cache = {}
+
+
+def workload(model, small=False, accum_fp64=True):
+    """Seeded sessions prepared by the oracle, plus the oracle's 50-iteration run."""
+    key = (model, small, accum_fp64)
+    if key not in _cache:
+        name, n, nq = CASES[model]
+        if small:
+            n, nq = SMALL[model], SMALL[model] // 10
+        s = synth.generate(synth.SynthConfig(n_sessions=n, n_queries=nq, vocab=40 * nq, model=name,
+                                             seed=100 + model + (7 if small else 0)))
+        ds = oracle.from_sessions(s.query, s.docs, s.clicks, 0.2)
+        tr = ds["train"]
+        om = oracle.OracleModel(model, tr["query"], tr["pidx"], tr["clicks"], ds["n_qd"], accum_fp64=accum_fp64)
+        om.iterate(ITERS)
+        tabs = [om.get_table(t) for t in range(om.n_tables())]
+        te = ds["test"]
+        ev = om.evaluate(te["pidx"], te["clicks"])
+        _cache[key] = (ds, tabs, ev)
+    return _cache[key]
+
+
+def sset(d):
+    return SessionSet(d["query"], d["docs"], d["clicks"], d["pidx"])
+
+
+def run_gpu(model, ds, iters=ITERS, use_graph=True, **opts):
+    cm = ClickModel(model, device=0, **opts)
+    cm.init_parameters(sset(ds["train"]), ds["n_qd"])
+    cm.em(iters, use_graph=use_graph)
+    return cm
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+@pytest.mark.parametrize("mode", [REDUCE_SEGMENTED, REDUCE_ATOMIC], ids=["segmented", "atomic"])
+def test_params_match_oracle_after_50_iterations(model, mode):
+    ds, tabs, _ = workload(model)
+    print(f"case model={model} mode={mode} n_train={len(ds['train']['query'])} n_qd={ds['n_qd']}", flush=True)
+    with run_gpu(model, ds, reduce_mode=mode) as cm:
+        for t, (onum, oden, oval) in enumerate(tabs):
+            val = cm.get_values(t)
+            num, den = cm.get_parameters(t)
+            err = float(np.max(np.abs(val - oval)))
+            assert err < TOL_PARAM, f"table {t}: max |value - oracle| = {err}"
+            # Denominators are sums of integers below 2^24 in both paths: exact.
+            if not cm.get_parameter_information()[t][1] or model in (PBM, UBM):
+                np.testing.assert_array_equal(den, oden)
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_params_match_reference_faithful_oracle(model):
+    ds, tabs, _ = workload(model, small=True, accum_fp64=False)
+    print(f"case model={model} small n_train={len(ds['train']['query'])} n_qd={ds['n_qd']}", flush=True)
+    with run_gpu(model, ds) as cm:
+        for t, (onum, oden, oval) in enumerate(tabs):
+            err = float(np.max(np.abs(cm.get_values(t) - oval)))
+            assert err < TOL_PARAM, f"table {t}: max |value - oracle| = {err}"
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_heldout_llh_and_perplexity_match_oracle(model):
+    ds, _, oev = workload(model)
+    with run_gpu(model, ds) as cm:
+        ev = cm.evaluate(sset(ds["test"]), per_session=True)
+    rel = abs(ev["llh_sum"] - oev["llh_sum_f64"]) / abs(oev["llh_sum_f64"])
+    assert rel < TOL_LL, f"LL rel diff {rel}"
+    prel = np.abs(ev["ppl_log2_sum"] - oev["ppl_log2_sum_f64"]) / np.abs(oev["ppl_log2_sum_f64"])
+    assert prel.max() < TOL_LL, f"perplexity rel diff {prel.max()}"
+    np.testing.assert_array_equal(ev["rank_count"], oev["rank_count"])
+    assert np.max(np.abs(ev["logp"] - oev["logp"])) < 1e-5
+    assert np.max(np.abs(ev["full"] - oev["full"])) < 1e-5
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_segmented_mode_is_deterministic_and_graph_equals_eager(model):
+    ds, _, _ = workload(model)
+    outs = []
+    for use_graph in (True, True, False):
+        with run_gpu(model, ds, iters=7, use_graph=use_graph) as cm:
+            outs.append([cm.get_parameters(t) for t in range(cm.n_tables())])
+    for other in outs[1:]:
+        for (n0, d0), (n1, d1) in zip(outs[0], other):
+            np.testing.assert_array_equal(n0, n1)
+            np.testing.assert_array_equal(d0, d1)
+
+
+@pytest.mark.parametrize("model", [PBM, DBN], ids=["PBM", "DBN"])
+def test_stepwise_api_equals_em_iterations(model):
+    ds, _, _ = workload(model)
+    with run_gpu(model, ds, iters=3, use_graph=False) as a:
+        ref = [a.get_parameters(t) for t in range(a.n_tables())]
+    cm = ClickModel(model, device=0)
+    cm.init_parameters(sset(ds["train"]), ds["n_qd"])
+    for _ in range(3):
+        cm.process_session()
+        cm.update_parameters()
+    for t in range(cm.n_tables()):
+        n, d = cm.get_parameters(t)
+        np.testing.assert_array_equal(n, ref[t][0])
+        np.testing.assert_array_equal(d, ref[t][1])
+    cm.close()
