@@ -1,0 +1,198 @@
+"""Benchmark: plain-EM DBN on 10M synthetic sessions (8M train), one MI355X.
+
+One "step" is one EM iteration (E-step + M-step) over the whole training set,
+replayed from a captured HIP graph with the sessions and tables resident in
+HBM.  Prints one JSON line:
+
+  value       sessions/s per iteration = N_train / mean iteration time
+  roofline    the E/M item kernel (the dominant kernel): algorithmic bytes per
+              launch / its mean duration (HIP events on the engine's stream)
+              against the 8 TB/s HBM peak
+  cpu_baseline  the CPU oracle (reference-faithful threading) on a bounded
+              sample of the same workload, on this host's cores
+
+Algorithmic bytes (per iteration): 42 B per training session (ten int32
+query-document indices + a 16-bit click mask) + 16 B per table entry and
+per-qd table (read and write a numerator/denominator pair), DBN has two
+per-qd tables.  The item kernel owns the entries of query groups it holds
+whole; entries of groups split across items are finalised by the merge
+kernel and are excluded from the item kernel's bytes.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ds, model, n_sample, iters, threads):
+    """Times the CPU oracle (reference-faithful threading: even E-step split,
+    query-aligned M-step ranges, mutex merge of the shared table) on a uniform
+    random sample of the training sessions, kept in query order."""
+    import numpy as np
+    import oracle
+    tr = ds.train
+    n = min(n_sample, len(tr))
+    idx = np.sort(np.random.default_rng(0).choice(len(tr), n, replace=False))
+    q = tr.query[idx]
+    p = tr.pidx[idx]
+    n_qd = int(p.max()) + 1
+    om = oracle.OracleModel(model, q, p, tr.clicks[idx], n_qd, n_threads=threads)
+    om.iterate(1)   # warm
+    te = tm = 0.0
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        om.estep()
+        t1 = time.perf_counter()
+        om.mstep()
+        t2 = time.perf_counter()
+        te += t1 - t0
+        tm += t2 - t1
+    dt = (te + tm) / iters
+    om.close()
+    return {"value": n / dt, "unit": "sessions/s/iter", "cores": threads, "kind": "port",
+            "sample": f"{n} of the training sessions (uniform sample, query order kept) x {iters} iterations, "
+                      f"{threads} threads: {dt:.3f} s/iter (E {te / iters:.3f} s, M {tm / iters:.3f} s)"}
+
+
+def calibration_copy(mib):
+    """One device-to-device copy of `mib` MiB (a plain copy kernel of known
+    size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    nbytes = C.c_size_t(mib << 20)
+    assert hip.hipMalloc(C.byref(a), nbytes) == 0 and hip.hipMalloc(C.byref(b), nbytes) == 0
+    assert hip.hipMemset(a, 1, nbytes) == 0
+    assert hip.hipMemcpy(b, a, nbytes, 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--sessions", type=int, default=10_000_000)
+    ap.add_argument("--queries", type=int, default=1_000_000)
+    ap.add_argument("--model", default="dbn", choices=["pbm", "ubm", "dbn", "ccm"])
+    ap.add_argument("--mode", default="segmented", choices=["segmented", "atomic"])
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--item-sessions", type=int, default=None)
+    ap.add_argument("--block", type=int, default=None)
+    ap.add_argument("--calib-mib", type=int, default=0, help="also run a D2D copy of this size")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "query-sharded design is described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from massiveclicks_amd import Dataset, ClickModel, synth
+    from massiveclicks_amd.model import PBM, UBM, DBN, CCM, REDUCE_ATOMIC, REDUCE_SEGMENTED
+
+    model = {"pbm": PBM, "ubm": UBM, "dbn": DBN, "ccm": CCM}[args.model]
+    n_tab = 2 if model == DBN else 1
+
+    t0 = time.perf_counter()
+    cfg = synth.SynthConfig(n_sessions=args.sessions, n_queries=args.queries, vocab=20 * args.queries,
+                            zipf_s=1.1, model=args.model, seed=3)
+    s = synth.generate(cfg)
+    log(f"generated {args.sessions} sessions in {time.perf_counter() - t0:.1f} s: {s.stats}")
+    t0 = time.perf_counter()
+    ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2)
+    del s
+    log(f"prepared: train {ds.n_train}, test {ds.n_test}, n_qd {ds.n_qd} in {time.perf_counter() - t0:.1f} s")
+
+    opts = {"reduce_mode": REDUCE_ATOMIC if args.mode == "atomic" else REDUCE_SEGMENTED}
+    if args.item_sessions:
+        opts["item_sessions"] = args.item_sessions
+    if args.block:
+        opts["block"] = args.block
+    cm = ClickModel(model, device=0, **opts)
+    t0 = time.perf_counter()
+    cm.init_from_dataset(ds)
+    lay = cm.layout_info()
+    log(f"init (layout + H2D) {time.perf_counter() - t0:.1f} s, layout {lay}")
+
+    # Warmup, then the timed region: K iterations replayed from one graph.
+    cm.em(args.warmup, use_graph=True)
+    ms_total = cm.em_graph_timed(args.steps)
+    ms_iter = ms_total / args.steps
+    # Per-kernel durations from events around each launch (eager launches).
+    kt = cm.em_timed(min(args.steps, 10))
+    log(f"graph: {ms_iter:.4f} ms/iter; eager: item kernel {kt['ms_estep']:.4f} ms, merge {kt['ms_mstep']:.4f} ms")
+
+    n_train = ds.n_train
+    value = n_train / (ms_iter * 1e-3)
+    n_shared = lay["n_shared_entries"]
+    item_bytes = n_train * 42 + 16 * n_tab * (ds.n_qd - (n_shared if args.mode == "segmented" else ds.n_qd))
+    if args.mode == "atomic":
+        item_bytes = n_train * 42
+    iter_bytes = n_train * 42 + 16 * n_tab * ds.n_qd
+    achieved = item_bytes / (kt["ms_estep"] * 1e-3)
+
+    ev = cm.evaluate(ds.test)
+    log(f"held-out LL {ev['log_likelihood']:.6f}, perplexity {ev['perplexity']:.6f}")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"{args.model}_{args.sessions // 1_000_000}m_traffic.json")
+    if os.path.exists(tfile) and args.mode == "segmented":
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(ds, model, args.cpu_sample, args.cpu_iters, threads)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "EM sessions/sec/iter (DBN, 10M sess)" if args.model == "dbn" else f"EM sessions/sec/iter ({args.model.upper()})",
+        "value": value,
+        "unit": "sessions/s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_iter,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.model.upper()} plain EM, {args.sessions} synthetic sessions "
+                               f"({n_train} train), {args.queries} Zipf(1.1) queries, pools of 20, "
+                               f"{args.mode} reduction",
+                   "sessions": args.sessions, "train_sessions": n_train, "n_qd": ds.n_qd,
+                   "iterations": args.steps, "reduce_mode": args.mode, "work_items": lay["n_items"]},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "em_item_kernel", "bytes_per_launch": item_bytes,
+                     "kernel_ms": kt["ms_estep"], "merge_kernel_ms": kt["ms_mstep"],
+                     "iteration_bytes": iter_bytes,
+                     "iteration_frac": iter_bytes / (ms_iter * 1e-3) / HBM_PEAK},
+        "cpu_baseline": cpu,
+        "heldout": {"log_likelihood": ev["log_likelihood"], "perplexity": ev["perplexity"]},
+    }
+    cm.close()
+    if args.calib_mib:
+        calibration_copy(args.calib_mib)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

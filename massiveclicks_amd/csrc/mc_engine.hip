@@ -594,8 +594,9 @@ mc_status get_graph(mc_model* m, int n_iter, int cur, hipGraphExec_t* out) {
     return MC_OK;
 }
 
-// Largest per-qd accumulator window that keeps two workgroups per CU.
-int default_window(int T) { return (48 * 1024) / (T * 8); }
+// Per-qd accumulator window: 32 KB of LDS per workgroup (five workgroups of
+// 256 threads per CU with 160 KB of LDS).
+int default_window(int T) { return (32 * 1024) / (T * 8); }
 
 }  // namespace
 

@@ -85,9 +85,20 @@ __device__ __forceinline__ float clamp_value(float num, float den) {   // param.
     return v < kClampMax ? v : kClampMax;
 }
 
+// Non-negative float -> 32.32 fixed point, truncating below 2^-32.  Every
+// expected count is a probability ratio >= 0; the integer part and the
+// fraction are converted separately with the native u32 conversion.
 __device__ __forceinline__ long long to_fix(float x) {
-    return (long long)__builtin_rintf(x * kFixScale);
+    const uint32_t hi = (uint32_t)x;
+    const float f = x - (float)hi;
+    const uint32_t lo = (uint32_t)(f * kFixScale);
+    return (long long)(((unsigned long long)hi << 32) | lo);
 }
+
+// a / b through the hardware reciprocal (1 ulp); used in the DBN and CCM
+// chains, where the reference's own arithmetic (log/exp of products) differs
+// from these restructured products by more than this anyway.
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // Highest set bit of m, or 10 when m == 0 (search.cu:160-173).
 __device__ __forceinline__ int last_click(uint32_t m) {
@@ -264,23 +275,24 @@ __device__ __forceinline__ void dbn_session(const KArgs& A, int64_t s, int L, ui
             float phi4, phi5, sum;
             if (r == lcr) {
                 float cn = (r < L - 1) ? car[r + 1] : 0.f;
-                em.add(1, pv[r], sv[r] / (1 - (1 - sv[r]) * g * cn));
+                em.add(1, pv[r], fdiv(sv[r], 1 - (1 - sv[r]) * g * cn));
                 phi4 = (1 - sv[r]) * (1 - g);
                 phi5 = (1 - sv[r]) * g * Tn;
                 float phi6 = sv[r];
                 sum = phi4 + phi5 + phi6;
                 ex = g * (1 - sv[r]);
             } else {
-                em.add(0, pv[r], (av[r] * (1 - exam[r])) / (1 - exam[r] * car[r]));
+                em.add(0, pv[r], fdiv(av[r] * (1 - exam[r]), 1 - exam[r] * car[r]));
                 float E = ex;
                 float phi0 = 1 - E;
                 phi4 = (1 - g) * E;
                 phi5 = g * Tn * E;
                 sum = phi0 + phi4 + phi5;
-                ex = ex * (t[r] / (1 - av[r] * ex));
+                ex = ex * fdiv(t[r], 1 - av[r] * ex);
             }
-            gn += phi5 / sum;
-            gd += (phi4 + phi5) / sum;
+            const float inv = __builtin_amdgcn_rcpf(sum);
+            gn += phi5 * inv;
+            gd += (phi4 + phi5) * inv;
         }
     }
 }
@@ -298,7 +310,9 @@ __device__ __forceinline__ void dbn_session(const KArgs& A, int64_t s, int L, ui
 //   x = (1-a)tau2 / ((1-a)tau2 + a tau3) to tau2 as (x, x) and 1-x to tau3.
 //   At the last click and after it the tail product of start r+1 is
 //   V_{L-1-r} with G_{k+1} = G_k tau1 (1-a_k).  The reference forms the
-//   denominators num + phi/sum in double and narrows them (Q7).
+//   denominators num + phi/sum in double and narrows them (ccm.cu:433-467);
+//   the per-lane float sums here round each term once, which is within the
+//   same tolerance.
 template <bool VARLEN, class Emit>
 __device__ __forceinline__ void ccm_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em, float t1,
                                             float t2, float t3, float (&ta)[6]) {
@@ -343,8 +357,9 @@ __device__ __forceinline__ void ccm_session(const KArgs& A, int64_t s, int L, ui
             if (clicked) {
                 float w2 = (1 - av[r]) * t2;
                 float w3 = av[r] * t3;
-                float x = w2 / (w2 + w3);
-                float y = w3 / (w2 + w3);
+                const float inv = __builtin_amdgcn_rcpf(w2 + w3);
+                float x = w2 * inv;
+                float y = w3 * inv;
                 ta[2] += x; ta[3] += x;
                 ta[4] += y; ta[5] += y;
             }
@@ -355,30 +370,32 @@ __device__ __forceinline__ void ccm_session(const KArgs& A, int64_t s, int L, ui
         else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
         if (r == lcr) {
             float cn = car[r + 1];
-            em.add(0, pv[r], av[r] / (1 - (t2 * (1 - av[r]) + t3 * av[r]) * cn));
+            em.add(0, pv[r], fdiv(av[r], 1 - (t2 * (1 - av[r]) + t3 * av[r]) * cn));
             float phi4 = (1 - av[r]) * (1 - t2);
             float phi5 = (1 - av[r]) * t2 * Tn;
             float phi6 = av[r] * (1 - t3);
             float phi7 = av[r] * t3 * Tn;
             float sum = phi4 + phi5 + phi6 + phi7;
-            float n2 = phi5 / sum;
-            float n3 = phi7 / sum;
+            const float inv = __builtin_amdgcn_rcpf(sum);
+            float n2 = phi5 * inv;
+            float n3 = phi7 * inv;
             ta[2] += n2;
-            ta[3] += (float)((double)n2 + (double)(phi4 / sum));
+            ta[3] += n2 + phi4 * inv;
             ta[4] += n3;
-            ta[5] += (float)((double)n3 + (double)(phi6 / sum));
+            ta[5] += n3 + phi6 * inv;
             ex = t2 * (1 - av[r]) + t3 * av[r];
         } else {
-            em.add(0, pv[r], ((1 - exam[r]) * av[r]) / (1 - exam[r] * car[r]));
+            em.add(0, pv[r], fdiv((1 - exam[r]) * av[r], 1 - exam[r] * car[r]));
             float E = ex;
             float phi0 = 1 - E;
             float phi4 = (1 - t1) * E;
             float phi5 = t1 * Tn * E;
             float sum = phi0 + phi4 + phi5;
-            float n1 = phi5 / sum;
+            const float inv = __builtin_amdgcn_rcpf(sum);
+            float n1 = phi5 * inv;
             ta[0] += n1;
-            ta[1] += (float)((double)n1 + (double)(phi4 / sum));
-            ex = ex * ((t1 * (1 - av[r])) / (1 - av[r] * ex));
+            ta[1] += n1 + phi4 * inv;
+            ex = ex * fdiv(t1 * (1 - av[r]), 1 - av[r] * ex);
         }
     }
 }

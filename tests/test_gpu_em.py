@@ -104,8 +104,27 @@ def test_heldout_llh_and_perplexity_match_oracle(model):
     prel = np.abs(ev["ppl_log2_sum"] - oev["ppl_log2_sum_f64"]) / np.abs(oev["ppl_log2_sum_f64"])
     assert prel.max() < TOL_LL, f"perplexity rel diff {prel.max()}"
     np.testing.assert_array_equal(ev["rank_count"], oev["rank_count"])
-    assert np.max(np.abs(ev["logp"] - oev["logp"])) < 1e-5
-    assert np.max(np.abs(ev["full"] - oev["full"])) < 1e-5
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_eval_kernel_matches_oracle_on_identical_parameters(model):
+    """Per-session conditional and full click probabilities on the same
+    parameters (the GPU's trained tables loaded into the oracle): only the
+    libm log differs (glibc logf/log2f vs the device's), so the bar is a few
+    float ulps."""
+    ds, _, _ = workload(model)
+    tr, te = ds["train"], ds["test"]
+    with run_gpu(model, ds, iters=20) as cm:
+        ev = cm.evaluate(sset(te), per_session=True)
+        om = oracle.OracleModel(model, tr["query"], tr["pidx"], tr["clicks"], ds["n_qd"])
+        for t in range(cm.n_tables()):
+            om.set_table(t, *cm.get_parameters(t))
+    oev = om.evaluate(te["pidx"], te["clicks"])
+    np.testing.assert_allclose(ev["full"], oev["full"], rtol=2e-6, atol=1e-7)
+    np.testing.assert_allclose(ev["logp"], oev["logp"], rtol=2e-6, atol=1e-7)
+    np.testing.assert_allclose(ev["llh"], oev["llh"], rtol=2e-6, atol=1e-7)
+    assert abs(ev["llh_sum"] - oev["llh_sum_f64"]) <= 1e-7 * abs(oev["llh_sum_f64"])
+    np.testing.assert_allclose(ev["ppl_log2_sum"], oev["ppl_log2_sum_f64"], rtol=1e-7)
 
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
