@@ -23,7 +23,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mclicks.h"
+#include "mclicks.h"
 #include "mc_internal.h"
 #include "mc_kernels.h"
 
@@ -62,27 +62,57 @@ template <> struct Shape<kUBM> { static constexpr int T = 1, K = 55, KACC = 55; 
 template <> struct Shape<kDBN> { static constexpr int T = 2, K = 1, KACC = 2; };
 template <> struct Shape<kCCM> { static constexpr int T = 1, K = 3, KACC = 6; };
 
-template <int MODEL, bool VARLEN, class Emit>
-__device__ __forceinline__ void run_session(const KArgs& A, int64_t s, Emit& em, const float* sh, float (&acc)[55]) {
-    const uint32_t m = A.mask[s];
-    const int L = VARLEN ? (int)A.len[s] : kL;
+template <int MODEL, bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void run_session(const int (&pv)[kL], int L, uint32_t m, const Src& src, Emit& em,
+                                            const float* sh, float (&acc)[55]) {
     const uint32_t mm = m & ((1u << L) - 1u);
     if (MODEL == kPBM) {
-        pbm_session(A, s, L, mm, em, sh, acc);
+        pbm_session(pv, L, mm, src, em, sh, acc);
     } else if (MODEL == kUBM) {
-        ubm_session(A, s, L, mm, em, sh, acc);
+        ubm_session(pv, L, mm, src, em, sh, acc);
     } else if (MODEL == kDBN) {
         if (mm == 0) return;   // every count of a click-free session is a constant
         float gn = 0.f, gd = 0.f;
-        dbn_session<VARLEN>(A, s, L, mm, em, sh[0], gn, gd);
+        dbn_session<VARLEN>(pv, L, mm, src, em, sh[0], gn, gd);
         acc[0] += gn;
         acc[1] += gd;
     } else {
         if (mm == 0) return;
         float ta[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        ccm_session<VARLEN>(A, s, L, mm, em, sh[0], sh[1], sh[2], ta);
+        ccm_session<VARLEN>(pv, L, mm, src, em, sh[0], sh[1], sh[2], ta);
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[k] += ta[k];
+    }
+}
+
+// Loads one session's click bits, length and query-document indices
+// (rank-major layout: rank r of session s at pidx[r * N + s]).
+template <bool VARLEN>
+__device__ __forceinline__ void load_session(const KArgs& A, int s, bool ok, uint32_t& m, int& L, int (&pv)[kL]) {
+    m = ok ? (uint32_t)A.mask[s] : 0u;
+    L = VARLEN ? (ok ? (int)A.len[s] : 1) : kL;
+#pragma unroll
+    for (int r = 0; r < kL; r++) pv[r] = ok ? A.pidx[(int64_t)r * A.N + s] : 0;
+}
+
+// One lane walks the sessions s0 + tid, s0 + tid + blockDim, ...; the
+// indices of the next session are in flight while one is processed.  The
+// first session is passed in already loaded.
+template <int MODEL, bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void run_item(const KArgs& A, const Item& it, const Src& src, Emit& em, const float* sh,
+                                         float (&acc)[55], uint32_t m, int L, int (&pv)[kL]) {
+    const int nthr = blockDim.x;
+    for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
+        const int sn = s + nthr;
+        uint32_t m2;
+        int L2;
+        int pv2[kL];
+        load_session<VARLEN>(A, sn, sn < it.s1, m2, L2, pv2);
+        run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
+        m = m2;
+        L = L2;
+#pragma unroll
+        for (int r = 0; r < kL; r++) pv[r] = pv2[r];
     }
 }
 
@@ -93,8 +123,22 @@ __device__ __forceinline__ long long wave_sum(long long v) {
     return v;
 }
 
-template <int MODEL, bool VARLEN>
-__global__ void __launch_bounds__(512) em_item_kernel(KArgs A) {
+// Fixed point (32.32) plus a float base, rounded to float.
+__device__ __forceinline__ float fix_to_float(float base, unsigned long long v) {
+    return (float)((double)base + (double)(long long)v * kFixInv);
+}
+
+// Session paths, one kernel instantiation each (each gets its own register
+// allocation): window items in LDS, items without a window (global 64-bit
+// fixed-point atomics), and the float-atomic mode.
+enum { kPathLds = 0, kPathGlobal = 1, kPathAtomic = 2 };
+
+// E-step and in-item M-step of one work item (item index item_offset +
+// blockIdx.x).  Dynamic LDS (window path): width x T fixed-point
+// accumulators, then width x T staged values of theta_t (both interleaved per
+// entry).
+template <int MODEL, bool VARLEN, int PATH>
+__global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) {
     constexpr int T = Shape<MODEL>::T;
     constexpr int K = Shape<MODEL>::K;
     constexpr int KACC = Shape<MODEL>::KACC;
@@ -102,13 +146,28 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A) {
     __shared__ float sh[K < 64 ? 64 : K];
     __shared__ long long red[KACC][8];
 
-    const Item it = A.items[blockIdx.x];
+    const Item it = A.items[item_offset + blockIdx.x];
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
     const int W = it.width;
-    if (A.mode == 0 && W > 0) {
-        for (int i = tid; i < T * W; i += nthr) lds[i] = 0ull;
+    constexpr bool win = PATH == kPathLds;
+    unsigned long long* lacc = lds;
+    float* lval = (float*)(lds + T * W);
+
+    // First session's loads go out before the window is staged.
+    uint32_t m0;
+    int L0;
+    int pv0[kL];
+    load_session<VARLEN>(A, it.s0 + tid, it.s0 + tid < it.s1, m0, L0, pv0);
+    for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
+    if (win) {
+        for (int i = tid; i < W; i += nthr) {
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                lacc[i * T + t] = 0ull;
+                lval[i * T + t] = A.tab[t].val_in[it.base + i];
+            }
+        }
     }
     __syncthreads();
 
@@ -116,18 +175,21 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A) {
 #pragma unroll
     for (int k = 0; k < 55; k++) acc[k] = 0.f;
 
-    if (A.mode == 1) {
+    const SrcGlobal gsrc{A.tab[0].val_in, A.tab[1].val_in};
+    if (PATH == kPathAtomic) {
         EmitGlobalFloat em{A.tab[0].gaccf, A.tab[1].gaccf};
-        for (int64_t s = it.s0 + tid; s < it.s1; s += nthr) run_session<MODEL, VARLEN>(A, s, em, sh, acc);
-    } else if (W > 0) {
-        EmitLds em{lds, it.base, W};
-        for (int64_t s = it.s0 + tid; s < it.s1; s += nthr) run_session<MODEL, VARLEN>(A, s, em, sh, acc);
+        run_item<MODEL, VARLEN>(A, it, gsrc, em, sh, acc, m0, L0, pv0);
+    } else if (PATH == kPathLds) {
+        EmitLds<T> em{lacc};
+        const SrcLds<T> lsrc{lval};
+        run_item<MODEL, VARLEN>(A, it, lsrc, em, sh, acc, m0, L0, pv0);
     } else {
         EmitGlobalFix em{A.tab[0].gacc, A.tab[1].gacc};
-        for (int64_t s = it.s0 + tid; s < it.s1; s += nthr) run_session<MODEL, VARLEN>(A, s, em, sh, acc);
+        run_item<MODEL, VARLEN>(A, it, gsrc, em, sh, acc, m0, L0, pv0);
     }
 
-    // Shared-table partials: fixed point, wave sums, then across waves.
+    // Shared tables: fixed point, wave sums, across waves, one atomic per
+    // accumulator and work item (integer adds: order-independent).
     const int wave = tid >> 6, lane = tid & 63, nwave = nthr >> 6;
 #pragma unroll
     for (int k = 0; k < KACC; k++) {
@@ -138,83 +200,89 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A) {
     for (int k = tid; k < KACC; k += nthr) {
         long long v = 0;
         for (int w = 0; w < nwave; w++) v += red[k][w];
-        A.sh_partial[(int64_t)k * A.n_items + blockIdx.x] = v;
+        if (v != 0) atomicAdd(&A.sh_acc[k], (unsigned long long)v);
     }
 
-    // Flush the LDS window.
-    if (A.mode == 0 && W > 0) {
+    // Flush the LDS window, four entries per thread per pass with the
+    // constant loads issued together.
+    if (win) {
         const bool interior = (it.flags & kItemInterior) != 0;
-        for (int i = tid; i < W; i += nthr) {
-            const int p = it.base + i;
+        for (int i0 = 0; i0 < W; i0 += 4 * nthr) {
+            int2 oc[4];
 #pragma unroll
-            for (int t = 0; t < T; t++) {
-                const unsigned long long v = lds[t * W + i];
-                if (interior) {
-                    const float2 bd = A.tab[t].base_den[p];
-                    const float num = (float)((double)bd.x + (double)(long long)v * kFixInv);
-                    A.tab[t].val_out[p] = clamp_value(num, bd.y);
-                    A.tab[t].num_out[p] = num;
-                } else if (v != 0ull) {
-                    atomicAdd(&A.tab[t].gacc[p], v);
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * nthr + tid;
+                oc[u] = (interior && i < W) ? A.counts[it.base + i] : make_int2(0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * nthr + tid;
+                if (i >= W) continue;
+                const int p = it.base + i;
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const unsigned long long v = lacc[i * T + t];
+                    if (interior) {
+                        const float2 bd = base_den<MODEL>(t, oc[u]);
+                        const float num = fix_to_float(bd.x, v);
+                        A.tab[t].val_out[p] = clamp_value(num, bd.y);
+                        if (A.write_num) A.tab[t].num_out[p] = num;
+                    } else if (v != 0ull) {
+                        atomicAdd(&A.tab[t].gacc[p], v);
+                    }
                 }
             }
         }
     }
 }
 
-// Merge kernel: blocks [0, K) finalise the shared tables, the rest finalise
-// per-qd entries accumulated in global memory (merged entries in segmented
-// mode, every entry in atomic mode).
+// Merge kernel: block 0 finalises the shared tables, the other blocks
+// finalise per-qd entries accumulated in global memory (entries merged across
+// items in segmented mode, every entry in atomic mode).
 template <int MODEL>
 __global__ void __launch_bounds__(256) em_merge_kernel(KArgs A) {
     constexpr int T = Shape<MODEL>::T;
     const int tid = threadIdx.x;
-    if ((int)blockIdx.x < A.K) {
-        const int k = blockIdx.x;
-        __shared__ long long red[2][4];
-        long long sn = 0, sd = 0;
-        const int an = A.sh_acc_map[2 * k], ad = A.sh_acc_map[2 * k + 1];
-        for (int i = tid; i < A.n_items; i += blockDim.x) {
-            if (an >= 0) sn += A.sh_partial[(int64_t)an * A.n_items + i];
-            if (ad >= 0) sd += A.sh_partial[(int64_t)ad * A.n_items + i];
-        }
-        sn = wave_sum(sn);
-        sd = wave_sum(sd);
-        if ((tid & 63) == 0) { red[0][tid >> 6] = sn; red[1][tid >> 6] = sd; }
-        __syncthreads();
-        if (tid == 0) {
-            long long tn = 0, td = 0;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) { tn += red[0][w]; td += red[1][w]; }
+    if (blockIdx.x == 0) {
+        const int k = tid;
+        if (k < A.K) {
+            const int an = A.sh_acc_map[2 * k], ad = A.sh_acc_map[2 * k + 1];
+            const long long tn = an >= 0 ? (long long)A.sh_acc[an] : 0;
+            const long long td = ad >= 0 ? (long long)A.sh_acc[ad] : 0;
             const float num = (float)(A.sh_base[2 * k] + (double)tn * kFixInv);
             const float den = (float)(A.sh_base[2 * k + 1] + (double)td * kFixInv);
             A.sh_num_out[k] = num;
             A.sh_den_out[k] = den;
             A.sh_val_out[k] = clamp_value(num, den);
         }
+        __syncthreads();
+        for (int j = tid; j < A.kacc; j += blockDim.x) A.sh_acc[j] = 0ull;
         return;
     }
-    const int64_t i = (int64_t)(blockIdx.x - A.K) * blockDim.x + tid;
+    const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
     if (A.mode == 1) {
         if (i >= A.n_qd) return;
+        const int2 oc = A.counts[i];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const float2 bd = A.tab[t].base_den[i];
+            const float2 bd = base_den<MODEL>(t, oc);
             const float v = A.tab[t].gaccf[i];
             const float num = (float)((double)bd.x + (double)v);
             A.tab[t].val_out[i] = clamp_value(num, bd.y);
-            A.tab[t].num_out[i] = num;
+            if (A.write_num) A.tab[t].num_out[i] = num;
             A.tab[t].gaccf[i] = 0.f;
         }
     } else {
         if (i >= A.n_shared) return;
         const int p = A.shared_idx[i];
+        const int2 oc = A.counts[p];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const float2 bd = A.tab[t].base_den[p];
+            const float2 bd = base_den<MODEL>(t, oc);
             const unsigned long long v = A.tab[t].gacc[p];
-            const float num = (float)((double)bd.x + (double)(long long)v * kFixInv);
+            const float num = fix_to_float(bd.x, v);
             A.tab[t].val_out[p] = clamp_value(num, bd.y);
-            A.tab[t].num_out[p] = num;
+            if (A.write_num) A.tab[t].num_out[p] = num;
             A.tab[t].gacc[p] = 0ull;
         }
     }
@@ -415,12 +483,14 @@ struct mc_model {
     uint8_t* d_len = nullptr;
     Item* d_items = nullptr;
     int n_items = 0;
+    int n_win_items = 0;      // items [0, n_win_items) have an LDS window
     int64_t max_item_sessions = 0;
     int32_t* d_shared_idx = nullptr;
     int64_t n_shared = 0;
     float* d_val[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [buf][t]
     float* d_num[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    float2* d_bd[2] = {nullptr, nullptr};
+    int2* d_counts = nullptr;
+    std::vector<int2> h_counts;   // host copy (occurrences, clicked) per entry
     unsigned long long* d_gacc[2] = {nullptr, nullptr};
     float* d_den_user[2] = {nullptr, nullptr};
     int den_state[3] = {0, 0, 0};  // per table: 0 prior (2), 1 user-set, 2 M-step constant
@@ -429,7 +499,7 @@ struct mc_model {
     float* d_sh_den[2] = {nullptr, nullptr};
     double* d_sh_base = nullptr;
     int32_t* d_sh_acc_map = nullptr;
-    long long* d_sh_partial = nullptr;
+    unsigned long long* d_sh_acc = nullptr;
     int cur = 0;
     bool estep_pending = false;
     size_t lds_bytes = 0;
@@ -499,7 +569,7 @@ mc_status fill(mc_model* m, float* p, float v, int64_t n) {
     return MC_OK;
 }
 
-KArgs make_args(mc_model* m, int cur) {
+KArgs make_args(mc_model* m, int cur, int write_num) {
     KArgs A;
     std::memset(&A, 0, sizeof(A));
     A.pidx = m->d_pidx;
@@ -514,17 +584,17 @@ KArgs make_args(mc_model* m, int cur) {
         A.tab[t].val_in = m->d_val[cur][tt];
         A.tab[t].val_out = m->d_val[cur ^ 1][tt];
         A.tab[t].num_out = m->d_num[cur ^ 1][tt];
-        A.tab[t].base_den = m->d_bd[tt];
         A.tab[t].gacc = m->d_gacc[tt];
         A.tab[t].gaccf = (float*)m->d_gacc[tt];
     }
     A.sh_val_in = m->d_sh_val[cur];
-    A.sh_partial = m->d_sh_partial;
+    A.sh_acc = m->d_sh_acc;
     A.shared_idx = m->d_shared_idx;
     A.n_shared = m->n_shared;
     A.n_qd = m->n_qd;
     A.n_tables = m->T;
-    A.write_num = 1;
+    A.write_num = write_num;
+    A.counts = m->d_counts;
     A.K = m->K;
     A.kacc = m->KACC;
     A.sh_base = m->d_sh_base;
@@ -535,16 +605,27 @@ KArgs make_args(mc_model* m, int cur) {
     return A;
 }
 
-template <int MODEL>
-void launch_item(mc_model* m, const KArgs& A) {
+template <int MODEL, int PATH>
+void launch_path(mc_model* m, const KArgs& A, int first, int count, size_t lds) {
+    if (count <= 0) return;
     if (m->varlen)
-        hipLaunchKernelGGL((em_item_kernel<MODEL, true>), dim3(m->n_items), dim3(m->block), m->lds_bytes, m->stream, A);
+        hipLaunchKernelGGL((em_item_kernel<MODEL, true, PATH>), dim3(count), dim3(m->block), lds, m->stream, A, first);
     else
-        hipLaunchKernelGGL((em_item_kernel<MODEL, false>), dim3(m->n_items), dim3(m->block), m->lds_bytes, m->stream, A);
+        hipLaunchKernelGGL((em_item_kernel<MODEL, false, PATH>), dim3(count), dim3(m->block), lds, m->stream, A, first);
 }
 
-mc_status launch_estep(mc_model* m, int cur) {
-    KArgs A = make_args(m, cur);
+template <int MODEL>
+void launch_item(mc_model* m, const KArgs& A) {
+    if (m->reduce_mode == MC_REDUCE_ATOMIC) {
+        launch_path<MODEL, kPathAtomic>(m, A, 0, m->n_items, 0);
+    } else {
+        launch_path<MODEL, kPathLds>(m, A, 0, m->n_win_items, m->lds_bytes);
+        launch_path<MODEL, kPathGlobal>(m, A, m->n_win_items, m->n_items - m->n_win_items, 0);
+    }
+}
+
+mc_status launch_estep(mc_model* m, int cur, int write_num) {
+    KArgs A = make_args(m, cur, write_num);
     switch (m->type) {
         case MC_PBM: launch_item<kPBM>(m, A); break;
         case MC_UBM: launch_item<kUBM>(m, A); break;
@@ -555,10 +636,10 @@ mc_status launch_estep(mc_model* m, int cur) {
     return MC_OK;
 }
 
-mc_status launch_mstep(mc_model* m, int cur) {
-    KArgs A = make_args(m, cur);
+mc_status launch_mstep(mc_model* m, int cur, int write_num) {
+    KArgs A = make_args(m, cur, write_num);
     int64_t n_entries = m->reduce_mode == MC_REDUCE_ATOMIC ? m->n_qd : m->n_shared;
-    unsigned grid = (unsigned)(m->K + (n_entries + 255) / 256);
+    unsigned grid = (unsigned)(1 + (n_entries + 255) / 256);
     switch (m->type) {
         case MC_PBM: hipLaunchKernelGGL(em_merge_kernel<kPBM>, dim3(grid), dim3(256), 0, m->stream, A); break;
         case MC_UBM: hipLaunchKernelGGL(em_merge_kernel<kUBM>, dim3(grid), dim3(256), 0, m->stream, A); break;
@@ -579,8 +660,9 @@ mc_status get_graph(mc_model* m, int n_iter, int cur, hipGraphExec_t* out) {
     int c = cur;
     mc_status st = MC_OK;
     for (int i = 0; i < n_iter && st == MC_OK; i++) {
-        st = launch_estep(m, c);
-        if (st == MC_OK) st = launch_mstep(m, c);
+        const int wn = i == n_iter - 1;   // numerators only for the final state
+        st = launch_estep(m, c, wn);
+        if (st == MC_OK) st = launch_mstep(m, c, wn);
         c ^= 1;
     }
     hipError_t e = hipStreamEndCapture(m->stream, &g);
@@ -594,9 +676,9 @@ mc_status get_graph(mc_model* m, int n_iter, int cur, hipGraphExec_t* out) {
     return MC_OK;
 }
 
-// Per-qd accumulator window: 32 KB of LDS per workgroup (five workgroups of
-// 256 threads per CU with 160 KB of LDS).
-int default_window(int T) { return (32 * 1024) / (T * 8); }
+// Per-qd window: 1024 entries per per-qd table (an 8-byte accumulator and
+// a staged 4-byte value each: 24 KB of LDS for the DBN).
+int default_window(int T) { (void)T; return 1024; }
 
 }  // namespace
 
@@ -777,21 +859,12 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
             occ[p]++;
             if ((mask[s] >> r) & 1u) clk[p]++;
         }
-    std::vector<float2> bd0((size_t)n_qd), bd1;
+    std::vector<int2> counts((size_t)n_qd);
     for (int64_t p = 0; p < n_qd; p++) {
-        const double c = (double)clk[p], o = (double)occ[p];
-        switch (m->type) {
-            case MC_CCM: bd0[p] = make_float2((float)(1.0 + c), (float)(2.0 + o + c)); break;
-            default: bd0[p] = make_float2((float)(1.0 + c), (float)(2.0 + o)); break;
-        }
+        MC_REQUIRE(occ[p] < INT32_MAX, "too many occurrences of one query-document pair");
+        counts[p] = make_int2((int)occ[p], (int)clk[p]);
     }
-    if (m->type == MC_DBN) {
-        bd1.resize((size_t)n_qd);
-        for (int64_t p = 0; p < n_qd; p++) {
-            const double c = (double)clk[p], u = (double)(occ[p] - clk[p]);
-            bd1[p] = make_float2((float)(1.0 + u), (float)(2.0 + c + 2.0 * u));
-        }
-    }
+    m->h_counts = counts;
     std::vector<double> sh_base(2 * (size_t)K, 0.0);
     std::vector<int32_t> sh_map(2 * (size_t)K, -1);
     switch (m->type) {
@@ -934,11 +1007,23 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     }
     m->n_items = (int)items.size();
     m->n_shared = (int64_t)shared_idx.size();
+    // Window items first, then items without a window (their own launch);
+    // within each launch the largest items go first (the hardware dispatches
+    // workgroups in order, so long items do not end up in the tail).
+    std::stable_partition(items.begin(), items.end(), [](const Item& it) { return it.width > 0; });
+    {
+        auto mid = std::find_if(items.begin(), items.end(), [](const Item& it) { return it.width == 0; });
+        auto bigger = [](const Item& a, const Item& b) { return (a.s1 - a.s0) > (b.s1 - b.s0); };
+        std::stable_sort(items.begin(), mid, bigger);
+        std::stable_sort(mid, items.end(), bigger);
+    }
+    m->n_win_items = 0;
+    for (const Item& it : items) m->n_win_items += (m->reduce_mode == MC_REDUCE_SEGMENTED && it.width > 0) ? 1 : 0;
     m->max_item_sessions = 0;
     for (const Item& it : items) m->max_item_sessions = std::max<int64_t>(m->max_item_sessions, it.s1 - it.s0);
     int max_width = 0;
     for (const Item& it : items) max_width = std::max(max_width, (int)it.width);
-    m->lds_bytes = m->reduce_mode == MC_REDUCE_SEGMENTED ? (size_t)max_width * T * 8 : 0;
+    m->lds_bytes = m->reduce_mode == MC_REDUCE_SEGMENTED ? (size_t)max_width * T * 12 : 0;
 
     // Reorder the sessions of each item by last-click rank (wavefront-uniform
     // chain lengths); the per-entry sums do not depend on the order.
@@ -953,12 +1038,18 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     std::vector<int32_t> h_pidx((size_t)n * kMaxSerp);
     std::vector<uint16_t> h_mask((size_t)n);
     std::vector<uint8_t> h_len((size_t)n);
-    for (int64_t j = 0; j < n; j++) {
-        const int64_t s = order[j];
-        for (int r = 0; r < kMaxSerp; r++)
-            h_pidx[(size_t)r * n + j] = r < L[s] ? pidx[s * kMaxSerp + r] : 0;
-        h_mask[j] = mask[s];
-        h_len[j] = L[s];
+    // Sessions of window items store their indices relative to the window
+    // base (the kernel addresses the LDS window with them); other sessions
+    // store absolute indices.
+    for (const Item& it : items) {
+        const int32_t off = (m->reduce_mode == MC_REDUCE_SEGMENTED && it.width > 0) ? it.base : 0;
+        for (int64_t j = it.s0; j < it.s1; j++) {
+            const int64_t s = order[j];
+            for (int r = 0; r < kMaxSerp; r++)
+                h_pidx[(size_t)r * n + j] = r < L[s] ? pidx[s * kMaxSerp + r] - off : 0;
+            h_mask[j] = mask[s];
+            h_len[j] = L[s];
+        }
     }
 
     MC_TRY(dalloc(m, &m->d_pidx, h_pidx.size() * 4));
@@ -971,7 +1062,6 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
             MC_TRY(dalloc(m, &m->d_val[b][t], (size_t)n_qd * 4));
             MC_TRY(dalloc(m, &m->d_num[b][t], (size_t)n_qd * 4));
         }
-        MC_TRY(dalloc(m, &m->d_bd[t], (size_t)n_qd * 8));
         MC_TRY(dalloc(m, &m->d_gacc[t], (size_t)n_qd * 8));
     }
     for (int b = 0; b < 2; b++) {
@@ -979,9 +1069,10 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
         MC_TRY(dalloc(m, &m->d_sh_num[b], (size_t)K * 4));
         MC_TRY(dalloc(m, &m->d_sh_den[b], (size_t)K * 4));
     }
+    MC_TRY(dalloc(m, &m->d_counts, (size_t)n_qd * 8));
     MC_TRY(dalloc(m, &m->d_sh_base, sh_base.size() * 8));
     MC_TRY(dalloc(m, &m->d_sh_acc_map, sh_map.size() * 4));
-    MC_TRY(dalloc(m, &m->d_sh_partial, (size_t)KACC * items.size() * 8));
+    MC_TRY(dalloc(m, &m->d_sh_acc, (size_t)KACC * 8));
 
     MC_HIP(hipMemcpyAsync(m->d_pidx, h_pidx.data(), h_pidx.size() * 4, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_mask, h_mask.data(), h_mask.size() * 2, hipMemcpyHostToDevice, m->stream));
@@ -991,10 +1082,10 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     if (!shared_idx.empty())
         MC_HIP(hipMemcpyAsync(m->d_shared_idx, shared_idx.data(), shared_idx.size() * 4, hipMemcpyHostToDevice,
                               m->stream));
-    MC_HIP(hipMemcpyAsync(m->d_bd[0], bd0.data(), bd0.size() * 8, hipMemcpyHostToDevice, m->stream));
-    if (T > 1) MC_HIP(hipMemcpyAsync(m->d_bd[1], bd1.data(), bd1.size() * 8, hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_counts, counts.data(), counts.size() * 8, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_sh_base, sh_base.data(), sh_base.size() * 8, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_sh_acc_map, sh_map.data(), sh_map.size() * 4, hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemsetAsync(m->d_sh_acc, 0, (size_t)KACC * 8, m->stream));
     for (int t = 0; t < T; t++) {
         MC_HIP(hipMemsetAsync(m->d_gacc[t], 0, (size_t)n_qd * 8, m->stream));
         for (int b = 0; b < 2; b++) {
@@ -1027,7 +1118,7 @@ mc_status mc_process_session(mc_model* m) {
     MC_REQUIRE(m && m->initialized, "model not initialised");
     MC_REQUIRE(!m->estep_pending, "E-step already pending; call mc_update_parameters");
     MC_HIP(hipSetDevice(m->device));
-    MC_TRY(launch_estep(m, m->cur));
+    MC_TRY(launch_estep(m, m->cur, 1));
     MC_HIP(hipStreamSynchronize(m->stream));
     m->estep_pending = true;
     return MC_OK;
@@ -1037,7 +1128,7 @@ mc_status mc_update_parameters(mc_model* m) {
     MC_REQUIRE(m && m->initialized, "model not initialised");
     MC_REQUIRE(m->estep_pending, "no pending E-step; call mc_process_session first");
     MC_HIP(hipSetDevice(m->device));
-    MC_TRY(launch_mstep(m, m->cur));
+    MC_TRY(launch_mstep(m, m->cur, 1));
     MC_HIP(hipStreamSynchronize(m->stream));
     m->cur ^= 1;
     m->estep_pending = false;
@@ -1058,8 +1149,9 @@ mc_status mc_em_iterations(mc_model* m, int n_iter, int use_graph) {
     } else {
         int c = m->cur;
         for (int i = 0; i < n_iter; i++) {
-            MC_TRY(launch_estep(m, c));
-            MC_TRY(launch_mstep(m, c));
+            const int wn = i == n_iter - 1;
+            MC_TRY(launch_estep(m, c, wn));
+            MC_TRY(launch_mstep(m, c, wn));
             c ^= 1;
         }
     }
@@ -1080,10 +1172,11 @@ mc_status mc_em_iterations_timed(mc_model* m, int n_iter, double* ms_total, doub
     int c = m->cur;
     MC_HIP(hipEventRecord(ev[0], m->stream));
     for (int i = 0; i < n_iter; i++) {
+        const int wn = i == n_iter - 1;
         MC_HIP(hipEventRecord(ev[3 * i + 1], m->stream));
-        MC_TRY(launch_estep(m, c));
+        MC_TRY(launch_estep(m, c, wn));
         MC_HIP(hipEventRecord(ev[3 * i + 2], m->stream));
-        MC_TRY(launch_mstep(m, c));
+        MC_TRY(launch_mstep(m, c, wn));
         MC_HIP(hipEventRecord(ev[3 * i + 3], m->stream));
         c ^= 1;
     }
@@ -1168,9 +1261,13 @@ mc_status mc_get_parameters(mc_model* m, int t, float* num, float* den) {
         } else if (m->den_state[t] == 1) {
             MC_HIP(hipMemcpy(den, m->d_den_user[t], n * 4, hipMemcpyDeviceToHost));
         } else {
-            std::vector<float2> bd((size_t)n);
-            MC_HIP(hipMemcpy(bd.data(), m->d_bd[t], n * 8, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; i++) den[i] = bd[i].y;
+            // Same arithmetic as base_den() in the kernels.
+            for (int64_t i = 0; i < n; i++) {
+                const float o = (float)m->h_counts[i].x, c = (float)m->h_counts[i].y;
+                if (m->type == MC_CCM) den[i] = 2.f + o + c;
+                else if (m->type == MC_DBN && t == 1) den[i] = 2.f + 2.f * o - c;
+                else den[i] = 2.f + o;
+            }
         }
     }
     return MC_OK;

@@ -10,7 +10,7 @@
 // sum), entries shared with other items (query groups split over several
 // items) are merged with one 64-bit integer atomic per entry and finalised by
 // the merge kernel.  The shared (rank-indexed) tables are reduced per work
-// item into a partial buffer that the merge kernel sums.
+// item and merged with one 64-bit integer atomic per accumulator.
 //
 // Expected counts that do not depend on the parameters (clicked ranks, the
 // (1,2) contributions of temporary slots the reference never writes, the
@@ -48,7 +48,6 @@ struct TableArgs {
     const float* val_in;      // value() of theta_t, [n_qd]
     float* val_out;           // value() of theta_{t+1}
     float* num_out;           // numerator of theta_{t+1}
-    const float2* base_den;   // constant (prior + fixed counts, denominator), [n_qd]
     unsigned long long* gacc; // fixed-point accumulators of shared entries, [n_qd]
     float* gaccf;             // float accumulators (atomic mode), [n_qd]
 };
@@ -62,8 +61,9 @@ struct KArgs {
     int32_t n_items;
     int32_t mode;             // 0 segmented, 1 float atomics
     TableArgs tab[2];
+    const int2* counts;       // per qd entry: (occurrences, clicked occurrences) in training
     const float* sh_val_in;   // shared table values of theta_t
-    long long* sh_partial;    // [kacc][n_items]
+    unsigned long long* sh_acc;   // [kacc] fixed-point accumulators of the shared tables
     // merge kernel
     const int32_t* shared_idx;   // entries merged across items
     int64_t n_shared;
@@ -79,6 +79,21 @@ struct KArgs {
     float* sh_num_out;
     float* sh_den_out;
 };
+
+// Parameter-independent part of the M-step of per-qd table t for an entry
+// with `o` occurrences of which `c` were clicked: (prior + constant expected
+// counts, denominator).  Prior (1, 2) (utils/definitions.h:42-43); every
+// occurrence adds 1 to the denominator, CCM clicks add 2 (ccm.cu:311-314);
+// clicked occurrences add exactly 1 to the attractiveness numerator; for the
+// DBN satisfaction, clicked ranks add 1 to the denominator and unclicked
+// ranks keep the (1,2) of an unwritten temporary slot (dbn.cu:350-378).
+template <int MODEL>
+__device__ __forceinline__ float2 base_den(int t, int2 oc) {
+    const float o = (float)oc.x, c = (float)oc.y;
+    if (MODEL == kCCM) return make_float2(1.f + c, 2.f + o + c);
+    if (MODEL == kDBN && t == 1) return make_float2(1.f + (o - c), 2.f + 2.f * o - c);
+    return make_float2(1.f + c, 2.f + o);
+}
 
 __device__ __forceinline__ float clamp_value(float num, float den) {   // param.cu:75-87
     float v = num / den;
@@ -108,11 +123,11 @@ __device__ __forceinline__ int last_click(uint32_t m) {
 // --------------------------------------------------------------------------
 // Emitters: where a per-qd expected count goes.
 // --------------------------------------------------------------------------
-struct EmitLds {            // fixed-point LDS window
-    unsigned long long* acc;   // [T][width]
-    int base, width;
+template <int T>
+struct EmitLds {            // fixed-point LDS window, relative indices
+    unsigned long long* acc;   // [width][T]
     __device__ __forceinline__ void add(int t, int p, float x) {
-        atomicAdd(&acc[t * width + (p - base)], (unsigned long long)to_fix(x));
+        atomicAdd(&acc[p * T + t], (unsigned long long)to_fix(x));
     }
 };
 struct EmitGlobalFix {      // fixed-point global (item without a window)
@@ -131,21 +146,41 @@ struct EmitGlobalFloat {    // float atomics (reference-like mode)
 };
 
 // --------------------------------------------------------------------------
-// Per-model session kernels.  Each takes the session index s, the rank
-// count L, the click bits mm (restricted to ranks < L) and emits the counts
-// that vary between iterations.
+// Value sources: where theta_t is read from.
+// --------------------------------------------------------------------------
+// Window items address LDS with the query-document index relative to the
+// window base (the device copy of the indices is stored that way), with the
+// T per-qd tables interleaved per entry.
+template <int T>
+struct SrcLds {             // the item's window of values, staged in LDS
+    const float* v;            // [width][T]
+    __device__ __forceinline__ float get(int t, int p) const { return v[p * T + t]; }
+    __device__ __forceinline__ float2 pair(int p) const { return reinterpret_cast<const float2*>(v)[p]; }
+};
+struct SrcGlobal {          // straight from HBM, absolute indices
+    const float* v0;
+    const float* v1;
+    __device__ __forceinline__ float get(int t, int p) const { return t == 0 ? v0[p] : v1[p]; }
+    __device__ __forceinline__ float2 pair(int p) const { return make_float2(v0[p], v1[p]); }
+};
+
+// --------------------------------------------------------------------------
+// Per-model session kernels.  Each takes the session's query-document
+// indices pv (ranks < L valid), the rank count L, the click bits mm
+// (restricted to ranks < L) and emits the counts that vary between
+// iterations.
 // --------------------------------------------------------------------------
 
 // PBM E-step, pbm.cu:191-231.  Clicked ranks contribute (1,1) to both tables
 // (constant); unclicked ranks contribute ((a-ae)/(1-ae), 1) and ((e-ae)/(1-ae), 1).
-template <class Emit>
-__device__ __forceinline__ void pbm_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em,
+template <class Src, class Emit>
+__device__ __forceinline__ void pbm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             const float* sh, float (&acc)[55]) {
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         if (r < L && !((mm >> r) & 1u)) {
-            int p = A.pidx[(int64_t)r * A.N + s];
-            float atr = A.tab[0].val_in[p];
+            const int p = pv[r];
+            float atr = src.get(0, p);
             float ex = sh[r];
             float ae = atr * ex;
             float na = (atr - ae) / (1 - ae);
@@ -172,8 +207,8 @@ __device__ __forceinline__ void ubm_slots(uint32_t mm, int (&slot)[kL]) {
 // before rank 2 -> slot 12; click at rank 2 -> rank 4 uses slot 10 + 2 = 12);
 // the reference writes tmp[slot][session] in rank order, so only the last
 // rank writing a slot contributes to it.
-template <class Emit>
-__device__ __forceinline__ void ubm_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em,
+template <class Src, class Emit>
+__device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             const float* sh, float (&acc)[55]) {
     int slot[kL];
     ubm_slots(mm, slot);
@@ -184,8 +219,8 @@ __device__ __forceinline__ void ubm_session(const KArgs& A, int64_t s, int L, ui
 #pragma unroll
             for (int q = r + 1; q < kL; q++) last_writer = last_writer && !(q < L && slot[q] == slot[r]);
             const int row = r * (r + 1) / 2;
-            int p = A.pidx[(int64_t)r * A.N + s];
-            float atr = A.tab[0].val_in[p];
+            const int p = pv[r];
+            float atr = src.get(0, p);
             float ex = sh[slot[r]];
             float ae = atr * ex;
             float na = (atr - ae) / (1 - ae);
@@ -228,71 +263,77 @@ __device__ __forceinline__ float pick10(const float (&v)[kL], int i) {
 //   the examination probability otherwise) cancel in the ratios.  The
 //   conditional examination after the last click follows
 //   E_{lcr+1} = g(1-s_lcr), E_{r+1} = E_r t_r / (1 - a_r E_r).
-template <bool VARLEN, class Emit>
-__device__ __forceinline__ void dbn_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em, float g,
-                                            float& gn, float& gd) {
+template <bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
+                                            float g, float& gn, float& gd) {
     const int lcr = last_click(mm);
-    float av[kL], sv[kL];
-    int pv[kL];
+    // Pass 1, ranks 0..L-1: exam[r] (compute_exm_car) and the no-click tail
+    // products V_k; only exam[r] for r > lcr and V_k for k < L - lcr are used.
+    float exam[kL], V[kL];
+    {
+        float e = 1.f, v = 1.f, G = 1.f;
 #pragma unroll
-    for (int r = 0; r < kL; r++) {
-        if (!VARLEN || r < L) {
-            pv[r] = A.pidx[(int64_t)r * A.N + s];
-            av[r] = A.tab[0].val_in[pv[r]];
-            sv[r] = A.tab[1].val_in[pv[r]];
-        } else {
-            pv[r] = 0; av[r] = 0.f; sv[r] = 0.f;
+        for (int r = 0; r < kL; r++) {
+            exam[r] = e;
+            V[r] = v;
+            if (!VARLEN || r < L) {
+                const float2 as = src.pair(pv[r]);
+                const float t = g * (1 - as.x);
+                e = e * (t + g * as.x * (1 - as.y));
+                v = v - as.x * G;
+                G = G * t;
+            }
         }
     }
-    float t[kL], exam[kL + 1], car[kL + 1];
-    exam[0] = 1.f;
+    // Pass 2, ranks L-1 down to lcr+1 (all unclicked): car[r] and the
+    // attractiveness posterior a(1-exam)/(1-exam car).  The parameters are
+    // read again from LDS instead of being kept live in registers across the
+    // passes (the compiler barrier stops the loads from being merged).
+    __asm__ volatile("" ::: "memory");
+    float car = 0.f;
+#pragma unroll
+    for (int r = kL - 1; r >= 0; r--) {
+        if ((!VARLEN || r < L) && r > lcr) {
+            const float a = src.get(0, pv[r]);
+            car = a + g * (1 - a) * car;
+            em.add(0, pv[r], fdiv(a * (1 - exam[r]), 1 - exam[r] * car));
+        }
+    }
+    // Pass 3, ranks lcr..L-1: satisfaction at the last click, gamma factors.
+    // After the last click the conditional examination probability is kept
+    // as a ratio E = u / v: E_{r+1} = E_r g(1-a_r) / (1 - a_r E_r) becomes
+    // u' = u g(1-a_r), v' = v - a_r u, and the factor ratios of an unclicked
+    // rank, phi5/sum = g T E / (1 - g E (1-T)) and (phi4+phi5)/sum =
+    // E (1-g+gT) / (1 - g E (1-T)), share one reciprocal of v - g u (1-T).
+    __asm__ volatile("" ::: "memory");
+    float u = 0.f, v = 1.f;
 #pragma unroll
     for (int r = 0; r < kL; r++) {
-        t[r] = g * (1 - av[r]);
-        exam[r + 1] = exam[r] * (t[r] + g * av[r] * (1 - sv[r]));
-    }
-    car[kL] = 0.f;
-#pragma unroll
-    for (int r = kL - 1; r >= 0; r--) car[r] = (!VARLEN || r < L) ? av[r] + t[r] * car[r + 1] : 0.f;
-
-    float V[kL];
-    V[0] = 1.f;
-    float G = 1.f;
-#pragma unroll
-    for (int k = 0; k < kL - 1; k++) {
-        V[k + 1] = V[k] - av[k] * G;
-        G = G * t[k];
-    }
-
-    float ex = 0.f;
-#pragma unroll
-    for (int r = 0; r < kL; r++) {
-        const bool live = (!VARLEN || r < L) && r >= lcr;
-        if (live) {
+        if ((!VARLEN || r < L) && r >= lcr) {
             float Tn;
             if (VARLEN) Tn = (r + 1 < L) ? pick10(V, L - 1 - r) : 1.f;
             else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
-            float phi4, phi5, sum;
             if (r == lcr) {
-                float cn = (r < L - 1) ? car[r + 1] : 0.f;
-                em.add(1, pv[r], fdiv(sv[r], 1 - (1 - sv[r]) * g * cn));
-                phi4 = (1 - sv[r]) * (1 - g);
-                phi5 = (1 - sv[r]) * g * Tn;
-                float phi6 = sv[r];
-                sum = phi4 + phi5 + phi6;
-                ex = g * (1 - sv[r]);
+                const float sv = src.get(1, pv[r]);
+                em.add(1, pv[r], fdiv(sv, 1 - (1 - sv) * g * car));
+                const float phi4 = (1 - sv) * (1 - g);
+                const float phi5 = (1 - sv) * g * Tn;
+                const float inv = __builtin_amdgcn_rcpf(phi4 + phi5 + sv);
+                gn += phi5 * inv;
+                gd += (phi4 + phi5) * inv;
+                u = g * (1 - sv);
+                v = 1.f;
             } else {
-                em.add(0, pv[r], fdiv(av[r] * (1 - exam[r]), 1 - exam[r] * car[r]));
-                float E = ex;
-                float phi0 = 1 - E;
-                phi4 = (1 - g) * E;
-                phi5 = g * Tn * E;
-                sum = phi0 + phi4 + phi5;
-                ex = ex * fdiv(t[r], 1 - av[r] * ex);
+                const float a = src.get(0, pv[r]);
+                const float inv = __builtin_amdgcn_rcpf(v - g * u * (1 - Tn));
+                gn += g * Tn * u * inv;
+                gd += (1 - g + g * Tn) * u * inv;
+                const float u2 = u * (g * (1 - a));
+                v = v - a * u;
+                u = u2;
+                // Keep the ratio representable after many small factors.
+                if (v < 0x1p-60f) { u *= 0x1p60f; v *= 0x1p60f; }
             }
-            const float inv = __builtin_amdgcn_rcpf(sum);
-            gn += phi5 * inv;
-            gd += (phi4 + phi5) * inv;
         }
     }
 }
@@ -313,19 +354,17 @@ __device__ __forceinline__ void dbn_session(const KArgs& A, int64_t s, int L, ui
 //   denominators num + phi/sum in double and narrows them (ccm.cu:433-467);
 //   the per-lane float sums here round each term once, which is within the
 //   same tolerance.
-template <bool VARLEN, class Emit>
-__device__ __forceinline__ void ccm_session(const KArgs& A, int64_t s, int L, uint32_t mm, Emit& em, float t1,
-                                            float t2, float t3, float (&ta)[6]) {
+template <bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
+                                            float t1, float t2, float t3, float (&ta)[6]) {
     const int lcr = last_click(mm);
     float av[kL];
-    int pv[kL];
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         if (!VARLEN || r < L) {
-            pv[r] = A.pidx[(int64_t)r * A.N + s];
-            av[r] = A.tab[0].val_in[pv[r]];
+            av[r] = src.get(0, pv[r]);
         } else {
-            pv[r] = 0; av[r] = 0.f;
+            av[r] = 0.f;
         }
     }
     float t[kL], exam[kL + 1], car[kL + 1];
