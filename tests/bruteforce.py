@@ -1,0 +1,118 @@
+"""Brute-force enumeration of click-model hidden states (float64).
+
+Each function enumerates every assignment of the hidden binary variables of
+a short result list (L <= 5) under the model's generative story and sums
+probabilities exactly.  These are independent of the oracle's recursions and
+are used to pin them.
+
+Generative stories (one query session, ranks 0..L-1):
+  PBM/UBM  C_r = A_r * E_r with A_r ~ Bern(a_r), E_r ~ Bern(e_r) independent.
+  DBN      E_0 = 1; C_r = E_r A_r; after a click S_r ~ Bern(s_r) and
+           E_{r+1} = (1 - S_r) G_r, otherwise E_{r+1} = E_r G_r, G_r ~ Bern(g).
+  CCM      E_0 = 1; C_r = E_r A_r; after a click E_{r+1} = H_r with
+           H_r ~ Bern(t2 (1 - a_r) + t3 a_r), otherwise E_{r+1} = E_r G_r with
+           G_r ~ Bern(t1).
+"""
+import itertools
+
+import numpy as np
+
+
+def _assignments(n_bits):
+    return np.array(list(itertools.product((0, 1), repeat=n_bits)), dtype=np.int8)
+
+
+def cascade_joint(model, a, s=None, g=None, taus=None):
+    """Enumerates hidden states; returns (clicks [M, L], A [M, L], E [M, L+1], S [M, L], prob [M])."""
+    L = len(a)
+    H = _assignments(3 * L)          # per rank: A, S (DBN) / unused (CCM), coin
+    A = H[:, 0:L]
+    Sx = H[:, L:2 * L]
+    G = H[:, 2 * L:3 * L]
+    M = H.shape[0]
+    E = np.zeros((M, L + 1), np.int8)
+    C = np.zeros((M, L), np.int8)
+    p = np.ones(M)
+    E[:, 0] = 1
+    for r in range(L):
+        p *= np.where(A[:, r] == 1, a[r], 1 - a[r])
+        C[:, r] = E[:, r] * A[:, r]
+        if model == "dbn":
+            p *= np.where(Sx[:, r] == 1, s[r], 1 - s[r])
+            p *= np.where(G[:, r] == 1, g, 1 - g)
+            E[:, r + 1] = np.where(C[:, r] == 1, (1 - Sx[:, r]) * G[:, r], E[:, r] * G[:, r])
+        else:
+            t1, t2, t3 = taus
+            # the S bits are unused by the CCM: weight them 1/2 each so the
+            # enumeration stays a probability distribution
+            p *= 0.5
+            h = t2 * (1 - a[r]) + t3 * a[r]
+            cont = np.where(C[:, r] == 1, np.where(G[:, r] == 1, h, 1 - h), np.where(G[:, r] == 1, t1, 1 - t1))
+            p *= cont
+            E[:, r + 1] = np.where(C[:, r] == 1, G[:, r], E[:, r] * G[:, r])
+    return C, A, E, Sx, p
+
+
+def pattern_prob(C, p, clicks, upto=None):
+    L = C.shape[1] if upto is None else upto
+    m = np.all(C[:, :L] == np.asarray(clicks[:L])[None, :], axis=1)
+    return p[m].sum()
+
+
+def conditional_click_probs(model, clicks, **kw):
+    """P(C_r = c_r | C_<r = c_<r) for each rank."""
+    C, _, _, _, p = cascade_joint(model, **kw)
+    out = []
+    for r in range(len(clicks)):
+        out.append(pattern_prob(C, p, clicks, r + 1) / pattern_prob(C, p, clicks, r))
+    return np.array(out)
+
+
+def marginal_click_probs(model, **kw):
+    """P(C_r = 1) for each rank."""
+    C, _, _, _, p = cascade_joint(model, **kw)
+    return (C * p[:, None]).sum(axis=0)
+
+
+def atr_posterior_no_later_click(model, r, **kw):
+    """P(A_r = 1 | C_j = 0 for every j >= r), marginalising over ranks < r."""
+    C, A, _, _, p = cascade_joint(model, **kw)
+    m = np.all(C[:, r:] == 0, axis=1)
+    return p[m & (A[:, r] == 1)].sum() / p[m].sum()
+
+
+def sat_posterior_last_click(r, clicks, **kw):
+    """DBN: P(S_r = 1 | clicks as given), r = the last clicked rank."""
+    C, _, _, S, p = cascade_joint("dbn", **kw)
+    m = np.all(C == np.asarray(clicks)[None, :], axis=1)
+    return p[m & (S[:, r] == 1)].sum() / p[m].sum()
+
+
+def ubm_marginal_click_probs(p_click):
+    """UBM: p_click(r, prev) is P(C_r = 1 | last click before r at prev, -1 = none).
+    Returns P(C_r = 1) by enumerating all click patterns."""
+    L = len(p_click)
+    out = np.zeros(L)
+    for pat in itertools.product((0, 1), repeat=L):
+        pr = 1.0
+        prev = -1
+        for r in range(L):
+            q = p_click[r](prev)
+            pr *= q if pat[r] else 1 - q
+            if pat[r]:
+                prev = r
+        out += pr * np.array(pat)
+    return out
+
+
+def pbm_posteriors(a, e):
+    """(P(A=1 | C=0), P(E=1 | C=0)) with C = A E, A, E independent."""
+    num_a = num_e = z = 0.0
+    for A in (0, 1):
+        for E in (0, 1):
+            pr = (a if A else 1 - a) * (e if E else 1 - e)
+            if A * E == 0:
+                z += pr
+                num_a += pr * A
+                num_e += pr * E
+    return num_a / z, num_e / z

@@ -1,0 +1,136 @@
+"""Dataset preparation (parse, split, sort, qd indices, test filtering):
+hand-derived expectations from the reference's rules, checked against both
+the CPU oracle and the engine's host code (mc_dataset_* in libmclicks.so,
+which runs on the CPU)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from massiveclicks_amd import Dataset, synth
+from massiveclicks_amd._lib import MCError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "parse_fixture.txt")
+
+
+def both_parse(path, max_sessions, test_share):
+    o = oracle.parse_raw(path, max_sessions, test_share)
+    d = Dataset.parse(path, max_sessions, test_share)
+    return o, d
+
+
+def test_parse_fixture_rules():
+    """dataset.cpp:224-305, search.cu:93-122:
+    * query lines (15 tokens) open a SERP; it is emitted when the next query
+      line arrives, so the SERP of query 300 (last in the file) is dropped;
+    * a click marks the first rank showing that document (doc 21 of query
+      200 is at ranks 0 and 3: rank 0 is marked); clicks on documents not in
+      the SERP (99, 5) change nothing;
+    * lines whose first token is not a number are skipped; lines with other
+      token counts are ignored (the 4-token line "3 1 X 5" is a click line).
+    Split of 3 sessions at 0.2: int(3 * 0.8f - 0.5) + 1 = 2 train, 1 test."""
+    o, d = both_parse(FIXTURE, -1, 0.2)
+    assert o["n_parsed"] == d.n_parsed == 3
+    assert len(d.train) == 2 and len(d.test) == 1
+    np.testing.assert_array_equal(d.train.query, [100, 200])
+    np.testing.assert_array_equal(d.train.clicks[0], [0, 0, 1, 0, 0, 0, 0, 0, 0, 0])
+    np.testing.assert_array_equal(d.train.clicks[1], [1, 0, 0, 0, 0, 0, 0, 0, 0, 0])
+    # first-occurrence indices; the repeated document 21 shares index 10
+    np.testing.assert_array_equal(d.train.pidx[0], np.arange(10))
+    np.testing.assert_array_equal(d.train.pidx[1], [10, 11, 12, 10, 13, 14, 15, 16, 17, 18])
+    assert d.n_qd == o["n_qd"] == 19
+    np.testing.assert_array_equal(d.test.query, [100])
+    np.testing.assert_array_equal(d.test.clicks[0], [0] * 9 + [1])
+    np.testing.assert_array_equal(d.test.pidx[0], np.arange(10))
+    for k in ("query", "docs", "clicks", "pidx"):
+        np.testing.assert_array_equal(o["train"][k], getattr(d.train, k))
+        np.testing.assert_array_equal(o["test"][k], getattr(d.test, k))
+
+
+def test_parse_stops_at_max_sessions():
+    """The read loop stops once max_sessions SERPs were emitted (dataset.cpp:250)."""
+    o, d = both_parse(FIXTURE, 2, 0.2)
+    assert o["n_parsed"] == d.n_parsed == 2
+    np.testing.assert_array_equal(d.train.query, [100, 200])
+    assert len(d.test) == 0
+
+
+@pytest.mark.parametrize("n,share,n_train", [
+    (1, 0.2, 1), (2, 0.2, 2), (3, 0.2, 2), (5, 0.2, 4), (10, 0.2, 8), (40000, 0.2, 32000),
+    (6, 0.25, 5), (10, 0.3, 7), (3, 0.5, 2), (7, 0.0, 7),
+])
+def test_split_rounding(n, share, n_train):
+    """n_train = int(n * (1 - share) - 0.5) + 1 with (1 - share) in float and
+    the product in float (dataset.cpp:390); worked by hand per case."""
+    q = np.arange(n, dtype=np.int32)
+    docs = np.arange(n * 10, dtype=np.int32).reshape(n, 10)
+    clicks = np.zeros((n, 10), np.uint8)
+    d = Dataset.from_sessions(q, docs, clicks, share)
+    o = oracle.from_sessions(q, docs, clicks, share)
+    assert len(d.train) == len(o["train"]["query"]) == n_train
+
+
+def test_test_set_filtering():
+    """assign_test_parameter_indices (dataset.cpp:349-374): a test session
+    with an unseen query, or with no document seen for its query, is
+    dropped; unseen documents of a kept session get index -1; order kept."""
+    q = np.array([7, 8, 999, 7, 8, 7], np.int32)
+    docs = np.array([np.arange(10) + 100, np.arange(10) + 200, np.arange(10) + 100,
+                     np.arange(10) + 500, np.r_[200, 201, np.arange(8) + 900], np.arange(10) + 105],
+                    np.int32)
+    clicks = np.zeros((6, 10), np.uint8)
+    share = 0.6   # int(6 * 0.4f - 0.5) + 1 = 2 train sessions
+    d = Dataset.from_sessions(q, docs, clicks, share)
+    o = oracle.from_sessions(q, docs, clicks, share)
+    assert len(d.train) == 2
+    np.testing.assert_array_equal(d.test.query, [8, 7])
+    np.testing.assert_array_equal(d.test.pidx[0], [10, 11] + [-1] * 8)
+    np.testing.assert_array_equal(d.test.pidx[1], [5, 6, 7, 8, 9] + [-1] * 5)
+    for k in ("query", "docs", "pidx"):
+        np.testing.assert_array_equal(o["test"][k], getattr(d.test, k))
+
+
+def test_train_order_is_libstdcxx_sort_of_equal_keys():
+    """The train set is ordered by std::sort on the query (dataset.cpp:19-20),
+    which is not stable: the committed fixture holds the permutation
+    libstdc++ produces for 3000 keys with ties (tests/golden/sort_ties.json,
+    made by tests/golden/make_sort_fixture.py); it differs from a stable sort."""
+    fx = json.load(open(os.path.join(HERE, "golden", "sort_ties.json")))
+    keys = np.array(fx["keys"], np.int32)
+    perm = np.array(fx["perm"])
+    assert fx["differs_from_stable"]
+    n = len(keys)
+    docs = (np.arange(n)[:, None] * 10 + np.arange(10)[None, :]).astype(np.int32)
+    clicks = np.zeros((n, 10), np.uint8)
+    d = Dataset.from_sessions(keys, docs, clicks, 0.0)
+    o = oracle.from_sessions(keys, docs, clicks, 0.0)
+    np.testing.assert_array_equal(d.train.docs[:, 0] // 10, perm)
+    np.testing.assert_array_equal(o["train"]["docs"][:, 0] // 10, perm)
+
+
+def test_synthetic_log_round_trip_matches_oracle(tmp_path):
+    """The 40k-session PBM workload written in the raw format and parsed by
+    both implementations: identical train/test sets and indices."""
+    s = synth.generate(synth.config_pbm_40k())
+    path = str(tmp_path / "pbm40k.txt")
+    synth.write_raw_log(s, path)
+    o, d = both_parse(path, 40000, 0.2)
+    assert d.n_parsed == 40000 and len(d.train) == 32000
+    for k in ("query", "docs", "clicks", "pidx"):
+        np.testing.assert_array_equal(o["train"][k], getattr(d.train, k))
+        np.testing.assert_array_equal(o["test"][k], getattr(d.test, k))
+    # sessions parsed back in log order = the generated ones
+    n_tr = len(d.train)
+    np.testing.assert_array_equal(np.sort(d.train.query), np.sort(s.query[:n_tr]))
+
+
+def test_parse_errors_are_reported():
+    with pytest.raises(MCError) as e:
+        Dataset.parse("/nonexistent/log.txt", -1, 0.2)
+    assert e.value.status == 4
+    with pytest.raises(MCError) as e:
+        Dataset.from_sessions(np.zeros(1, np.int32), np.zeros((1, 10), np.int32), np.zeros((1, 10), np.uint8), -0.5)
+    assert e.value.status == 1
