@@ -42,7 +42,7 @@ typedef enum {
 enum {
     /* 0 = deterministic segmented reduction (default), 1 = float atomics. */
     MC_OPT_REDUCE_MODE = 0,
-    /* Target sessions per workgroup work item (default 2048). */
+    /* Target sessions per workgroup work item (default 8192). */
     MC_OPT_ITEM_SESSIONS = 1,
     /* Per-qd accumulator window per work item, in entries (default: auto). */
     MC_OPT_ITEM_WINDOW = 2,
@@ -50,7 +50,11 @@ enum {
      * wavefronts run uniform chains (default 1). */
     MC_OPT_SORT_BY_LCR = 3,
     /* Workgroup size of the E/M kernel: 64, 128, 256 or 512 (default 256). */
-    MC_OPT_BLOCK = 4
+    MC_OPT_BLOCK = 4,
+    /* 1 = sessions that are the only session of their query (and show
+     * distinct documents) update their entries directly, without a
+     * reduction (default 0). */
+    MC_OPT_DIRECT = 5
 };
 enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };
 

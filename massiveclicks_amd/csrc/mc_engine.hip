@@ -85,30 +85,85 @@ __device__ __forceinline__ void run_session(const int (&pv)[kL], int L, uint32_t
     }
 }
 
+// Buffer descriptors of the session stream: 32-bit offsets (no 64-bit
+// address arithmetic per load) and hardware bounds checks (a prefetch past
+// the last session reads zeros).
+struct Stream {
+    __amdgpu_buffer_rsrc_t pidx, mask, len;
+    uint32_t row_bytes;
+};
+
+__device__ __forceinline__ Stream make_stream(const KArgs& A) {
+    constexpr int kRawDwordFlags = 0x00020000;   // raw 32-bit buffer, no format conversion
+    Stream st;
+    st.pidx = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx, 0, (int)(uint32_t)(A.N * 40), kRawDwordFlags);
+    st.mask = __builtin_amdgcn_make_buffer_rsrc((void*)A.mask, 0, (int)(uint32_t)(A.N * 2), kRawDwordFlags);
+    st.len = A.len ? __builtin_amdgcn_make_buffer_rsrc((void*)A.len, 0, (int)(uint32_t)A.N, kRawDwordFlags) : st.mask;
+    st.row_bytes = (uint32_t)(A.N * 4);
+    return st;
+}
+
 // Loads one session's click bits, length and query-document indices
 // (rank-major layout: rank r of session s at pidx[r * N + s]).
 template <bool VARLEN>
-__device__ __forceinline__ void load_session(const KArgs& A, int s, bool ok, uint32_t& m, int& L, int (&pv)[kL]) {
-    m = ok ? (uint32_t)A.mask[s] : 0u;
-    L = VARLEN ? (ok ? (int)A.len[s] : 1) : kL;
+__device__ __forceinline__ void load_session(const Stream& st, int s, uint32_t& m, int& L, int (&pv)[kL]) {
+    const uint32_t o = (uint32_t)s;
+    m = __builtin_amdgcn_raw_buffer_load_b16(st.mask, o * 2u, 0, 0);
+    L = VARLEN ? (int)__builtin_amdgcn_raw_buffer_load_b8(st.len, o, 0, 0) : kL;
 #pragma unroll
-    for (int r = 0; r < kL; r++) pv[r] = ok ? A.pidx[(int64_t)r * A.N + s] : 0;
+    for (int r = 0; r < kL; r++) pv[r] = (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, o * 4u, r * st.row_bytes, 0);
 }
 
 // One lane walks the sessions s0 + tid, s0 + tid + blockDim, ...; the
 // indices of the next session are in flight while one is processed.  The
 // first session is passed in already loaded.
 template <int MODEL, bool VARLEN, class Src, class Emit>
-__device__ __forceinline__ void run_item(const KArgs& A, const Item& it, const Src& src, Emit& em, const float* sh,
+__device__ __forceinline__ void run_item(const Stream& st, const Item& it, const Src& src, Emit& em, const float* sh,
                                          float (&acc)[55], uint32_t m, int L, int (&pv)[kL]) {
     const int nthr = blockDim.x;
     for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
-        const int sn = s + nthr;
         uint32_t m2;
         int L2;
         int pv2[kL];
-        load_session<VARLEN>(A, sn, sn < it.s1, m2, L2, pv2);
+        load_session<VARLEN>(st, s + nthr, m2, L2, pv2);
         run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
+        m = m2;
+        L = L2;
+#pragma unroll
+        for (int r = 0; r < kL; r++) pv[r] = pv2[r];
+    }
+}
+
+// Direct items: every session is the only one of its query, so its entries
+// receive no other counts; the lane writes theta_{t+1} of its entries itself.
+template <int MODEL, bool VARLEN, class Src>
+__device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st, const Item& it, const Src& src,
+                                                const float* sh, float (&acc)[55], uint32_t m, int L,
+                                                int (&pv)[kL]) {
+    constexpr int T = Shape<MODEL>::T;
+    const int nthr = blockDim.x;
+    for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
+        uint32_t m2;
+        int L2;
+        int pv2[kL];
+        load_session<VARLEN>(st, s + nthr, m2, L2, pv2);
+        EmitRegs em;
+        em.clear();
+        run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (!VARLEN || r < L) {
+                const int p = pv[r];
+                const int2 oc = A.counts[p];
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const float2 bd = base_den<MODEL>(t, oc);
+                    const float num = bd.x + em.x[t][r];
+                    A.tab[t].val_out[p] = clamp_value(num, bd.y);
+                    if (A.write_num) A.tab[t].num_out[p] = num;
+                }
+            }
+        }
         m = m2;
         L = L2;
 #pragma unroll
@@ -131,7 +186,7 @@ __device__ __forceinline__ float fix_to_float(float base, unsigned long long v) 
 // Session paths, one kernel instantiation each (each gets its own register
 // allocation): window items in LDS, items without a window (global 64-bit
 // fixed-point atomics), and the float-atomic mode.
-enum { kPathLds = 0, kPathGlobal = 1, kPathAtomic = 2 };
+enum { kPathLds = 0, kPathGlobal = 1, kPathAtomic = 2, kPathDirect = 3 };
 
 // E-step and in-item M-step of one work item (item index item_offset +
 // blockIdx.x).  Dynamic LDS (window path): width x T fixed-point
@@ -155,10 +210,11 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     float* lval = (float*)(lds + T * W);
 
     // First session's loads go out before the window is staged.
+    const Stream st = make_stream(A);
     uint32_t m0;
     int L0;
     int pv0[kL];
-    load_session<VARLEN>(A, it.s0 + tid, it.s0 + tid < it.s1, m0, L0, pv0);
+    load_session<VARLEN>(st, it.s0 + tid, m0, L0, pv0);
     for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
     if (win) {
         for (int i = tid; i < W; i += nthr) {
@@ -178,14 +234,16 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     const SrcGlobal gsrc{A.tab[0].val_in, A.tab[1].val_in};
     if (PATH == kPathAtomic) {
         EmitGlobalFloat em{A.tab[0].gaccf, A.tab[1].gaccf};
-        run_item<MODEL, VARLEN>(A, it, gsrc, em, sh, acc, m0, L0, pv0);
+        run_item<MODEL, VARLEN>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLds) {
         EmitLds<T> em{lacc};
         const SrcLds<T> lsrc{lval};
-        run_item<MODEL, VARLEN>(A, it, lsrc, em, sh, acc, m0, L0, pv0);
+        run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+    } else if (PATH == kPathDirect) {
+        run_item_direct<MODEL, VARLEN>(A, st, it, gsrc, sh, acc, m0, L0, pv0);
     } else {
         EmitGlobalFix em{A.tab[0].gacc, A.tab[1].gacc};
-        run_item<MODEL, VARLEN>(A, it, gsrc, em, sh, acc, m0, L0, pv0);
+        run_item<MODEL, VARLEN>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
     }
 
     // Shared tables: fixed point, wave sums, across waves, one atomic per
@@ -435,13 +493,20 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
     }
 }
 
-// Sums block partials in a fixed order (one block, deterministic).
-__global__ void __launch_bounds__(64) eval_reduce_kernel(const double* partial, int n_blocks, double* out) {
-    const int k = threadIdx.x;
-    if (k >= 21) return;
+// Sums block partials: one block per output, each thread a fixed strided
+// subset, then a fixed tree; the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) eval_reduce_kernel(const double* partial, int n_blocks, double* out) {
+    __shared__ double red[256];
+    const int k = blockIdx.x, tid = threadIdx.x;
     double w = 0.0;
-    for (int b = 0; b < n_blocks; b++) w += partial[(int64_t)b * 21 + k];
-    out[k] = w;
+    for (int b = tid; b < n_blocks; b += 256) w += partial[(int64_t)b * 21 + k];
+    red[tid] = w;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) red[tid] += red[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) out[k] = red[0];
 }
 
 // Fills a float array with a constant (used instead of memsets so that the
@@ -468,10 +533,11 @@ struct mc_model {
     hipStream_t stream = nullptr;
     // options
     int reduce_mode = MC_REDUCE_SEGMENTED;
-    int item_sessions = 2048;
+    int item_sessions = 8192;
     int item_window = 0;   // 0 = auto
     int sort_by_lcr = 1;
     int block = 256;
+    int direct = 0;
     // shape
     int T = 1, K = 10, KACC = 10;
     int64_t N = 0, n_qd = 0;
@@ -483,7 +549,9 @@ struct mc_model {
     uint8_t* d_len = nullptr;
     Item* d_items = nullptr;
     int n_items = 0;
-    int n_win_items = 0;      // items [0, n_win_items) have an LDS window
+    int n_win_items = 0;      // items [0, n_win_items) have an LDS window,
+    int n_glob_items = 0;     // then items merged with global atomics,
+    int n_dir_items = 0;      // then direct items (single-session groups)
     int64_t max_item_sessions = 0;
     int32_t* d_shared_idx = nullptr;
     int64_t n_shared = 0;
@@ -620,7 +688,8 @@ void launch_item(mc_model* m, const KArgs& A) {
         launch_path<MODEL, kPathAtomic>(m, A, 0, m->n_items, 0);
     } else {
         launch_path<MODEL, kPathLds>(m, A, 0, m->n_win_items, m->lds_bytes);
-        launch_path<MODEL, kPathGlobal>(m, A, m->n_win_items, m->n_items - m->n_win_items, 0);
+        launch_path<MODEL, kPathGlobal>(m, A, m->n_win_items, m->n_glob_items, 0);
+        launch_path<MODEL, kPathDirect>(m, A, m->n_win_items + m->n_glob_items, m->n_dir_items, 0);
     }
 }
 
@@ -803,6 +872,9 @@ mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
         case MC_OPT_SORT_BY_LCR:
             m->sort_by_lcr = value != 0;
             break;
+        case MC_OPT_DIRECT:
+            m->direct = value != 0;
+            break;
         case MC_OPT_BLOCK:
             MC_REQUIRE(value == 64 || value == 128 || value == 256 || value == 512, "block must be 64/128/256/512");
             m->block = (int)value;
@@ -817,7 +889,7 @@ mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
 mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const int32_t* pidx,
                              const uint8_t* clicks, const uint8_t* len, int64_t n_qd) {
     MC_REQUIRE(m && query && pidx && clicks, "null argument");
-    MC_REQUIRE(n >= 1 && n <= INT32_MAX - 1, "session count out of range");
+    MC_REQUIRE(n >= 1 && n * 40 < (int64_t)UINT32_MAX, "session count out of range (at most 107,374,182)");
     MC_REQUIRE(n_qd >= 1 && n_qd <= INT32_MAX - 1, "n_qd out of range");
     MC_HIP(hipSetDevice(m->device));
     if (m->initialized) release(m);
@@ -920,12 +992,23 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     }
 
     // ---- work items -----------------------------------------------------------
+    // An item is a set of whole query groups, or a slice of one large group,
+    // whose sessions are laid out contiguously in device order.  Kinds:
+    //   window  groups packed under an entry window of at most W entries (LDS);
+    //   global  slices of a group whose entries exceed W (global atomics);
+    //   direct  single-session groups with distinct documents: each session
+    //           owns its entries, so no reduction is needed at all.
     const int W = m->item_window > 0 ? m->item_window : default_window(T);
     const int S = m->item_sessions;
-    std::vector<Item> items;
+    enum { kKindWindow = 0, kKindGlobal = 1, kKindDirect = 2 };
+    struct Spec {
+        std::vector<std::pair<int64_t, int64_t>> ranges;
+        int64_t count = 0;
+        int32_t base = 0, width = 0, flags = 0;
+        int kind = kKindWindow;
+    };
+    std::vector<Spec> specs;
     std::vector<int32_t> shared_idx;
-    std::vector<int64_t> order((size_t)n);   // device position -> input session
-    std::iota(order.begin(), order.end(), 0);
     bool structured = m->reduce_mode == MC_REDUCE_SEGMENTED;
     if (structured) {
         // Query groups: runs of equal query ids.  The segmented reduction needs
@@ -951,43 +1034,106 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
             s = e;
         }
         if (structured) {
-            Item cur{0, 0, 0, 0, kItemInterior, 0};
+            auto distinct = [&](int64_t s) {
+                for (int r = 0; r < L[s]; r++)
+                    for (int q = r + 1; q < L[s]; q++)
+                        if (pidx[s * kMaxSerp + r] == pidx[s * kMaxSerp + q]) return false;
+                return true;
+            };
+            // Direct items are short (two sessions per lane): there are few
+            // such sessions and each is independent, so many small
+            // workgroups spread them over the whole chip.
+            const int64_t S_dir = 2 * (int64_t)m->block;
+            Spec cur, dir;
             bool open = false;
-            auto close = [&]() { if (open) { items.push_back(cur); open = false; } };
+            auto close = [&]() { if (open) { specs.push_back(cur); open = false; } };
+            auto close_dir = [&]() {
+                if (dir.count) { dir.kind = kKindDirect; dir.flags = kItemDirect; specs.push_back(dir); dir = Spec(); }
+            };
             for (const Group& g : groups) {
                 const int64_t gs = g.s1 - g.s0;
                 const int64_t gw = g.hi - g.lo;
+                if (m->direct && gs == 1 && distinct(g.s0)) {
+                    dir.ranges.push_back({g.s0, g.s1});
+                    if (++dir.count == S_dir) close_dir();
+                    continue;
+                }
                 if (gs <= S && gw <= W) {
-                    if (open && (cur.s1 - cur.s0) + gs <= S && (g.hi - cur.base) <= W) {
-                        cur.s1 = (int32_t)g.s1;
+                    if (open && cur.count + gs <= S && (g.hi - cur.base) <= W) {
+                        cur.ranges.push_back({g.s0, g.s1});
+                        cur.count += gs;
                         cur.width = g.hi - cur.base;
                     } else {
                         close();
-                        cur = Item{(int32_t)g.s0, (int32_t)g.s1, g.lo, (int32_t)gw, kItemInterior, 0};
+                        cur = Spec();
+                        cur.ranges.push_back({g.s0, g.s1});
+                        cur.count = gs;
+                        cur.base = g.lo;
+                        cur.width = (int32_t)gw;
+                        cur.flags = kItemInterior;
                         open = true;
                     }
                 } else {
                     close();
                     const int64_t n_slices = (gs + S - 1) / S;
                     for (int64_t k = 0; k < n_slices; k++) {
+                        Spec sl;
                         const int64_t a = g.s0 + gs * k / n_slices, b = g.s0 + gs * (k + 1) / n_slices;
-                        items.push_back(Item{(int32_t)a, (int32_t)b, g.lo, gw <= W ? (int32_t)gw : 0, 0, 0});
+                        sl.ranges.push_back({a, b});
+                        sl.count = b - a;
+                        sl.base = g.lo;
+                        sl.width = gw <= W ? (int32_t)gw : 0;
+                        sl.kind = gw <= W ? kKindWindow : kKindGlobal;
+                        specs.push_back(sl);
                     }
                     for (int32_t p = g.lo; p < g.hi; p++) shared_idx.push_back(p);
                 }
             }
             close();
+            close_dir();
         }
     }
     if (!structured) {
         // Float atomics over uniform chunks; the merge pass finalises every entry.
         m->reduce_mode = MC_REDUCE_ATOMIC;
-        items.clear();
+        specs.clear();
         shared_idx.clear();
-        for (int64_t a = 0; a < n; a += S) items.push_back(Item{(int32_t)a, (int32_t)std::min<int64_t>(n, a + S), 0, 0, 0, 0});
+        for (int64_t a = 0; a < n; a += S) {
+            Spec c;
+            c.ranges.push_back({a, std::min<int64_t>(n, a + S)});
+            c.count = c.ranges[0].second - a;
+            c.kind = kKindGlobal;
+            specs.push_back(c);
+        }
     }
-    // Every session of an item with a window must index inside that window,
-    // and items must tile [0, n) in order: the kernel relies on both.
+    // Launch order: window items, then global items, then direct items; the
+    // largest items of each kind first (workgroups are dispatched in order, so
+    // long items do not end up in the tail).
+    std::stable_sort(specs.begin(), specs.end(), [](const Spec& a, const Spec& b) {
+        return a.kind != b.kind ? a.kind < b.kind : a.count > b.count;
+    });
+
+    // Device order: each item's sessions contiguous, sorted by last-click rank
+    // inside the item (wavefront-uniform chain lengths; the per-entry sums do
+    // not depend on the order).
+    std::vector<Item> items;
+    std::vector<int64_t> order;
+    order.reserve((size_t)n);
+    m->n_win_items = m->n_glob_items = m->n_dir_items = 0;
+    for (const Spec& sp : specs) {
+        const int64_t j0 = (int64_t)order.size();
+        for (const auto& rg : sp.ranges)
+            for (int64_t s = rg.first; s < rg.second; s++) order.push_back(s);
+        if (m->sort_by_lcr)
+            std::stable_sort(order.begin() + j0, order.end(), [&](int64_t a, int64_t b) { return lcr[a] < lcr[b]; });
+        items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, 0});
+        if (sp.kind == kKindWindow && m->reduce_mode == MC_REDUCE_SEGMENTED) m->n_win_items++;
+        else if (sp.kind == kKindDirect) m->n_dir_items++;
+        else m->n_glob_items++;
+    }
+    MC_REQUIRE((int64_t)order.size() == n, "internal: work items do not cover the sessions");
+    // Every session of a window item must index inside that window (the
+    // kernel addresses LDS with it); items must tile [0, n).
     {
         int64_t expect = 0;
         for (const Item& it : items) {
@@ -996,43 +1142,24 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
             if (it.width > 0) {
                 MC_REQUIRE(it.base >= 0 && (int64_t)it.base + it.width <= n_qd && it.width <= W,
                            "internal: item window out of range");
-                for (int64_t s = it.s0; s < it.s1; s++)
+                for (int64_t j = it.s0; j < it.s1; j++) {
+                    const int64_t s = order[j];
                     for (int r = 0; r < L[s]; r++) {
                         const int32_t p = pidx[s * kMaxSerp + r];
                         MC_REQUIRE(p >= it.base && p < it.base + it.width, "internal: session outside its item window");
                     }
+                }
             }
         }
         MC_REQUIRE(expect == n, "internal: work items do not cover the sessions");
     }
     m->n_items = (int)items.size();
     m->n_shared = (int64_t)shared_idx.size();
-    // Window items first, then items without a window (their own launch);
-    // within each launch the largest items go first (the hardware dispatches
-    // workgroups in order, so long items do not end up in the tail).
-    std::stable_partition(items.begin(), items.end(), [](const Item& it) { return it.width > 0; });
-    {
-        auto mid = std::find_if(items.begin(), items.end(), [](const Item& it) { return it.width == 0; });
-        auto bigger = [](const Item& a, const Item& b) { return (a.s1 - a.s0) > (b.s1 - b.s0); };
-        std::stable_sort(items.begin(), mid, bigger);
-        std::stable_sort(mid, items.end(), bigger);
-    }
-    m->n_win_items = 0;
-    for (const Item& it : items) m->n_win_items += (m->reduce_mode == MC_REDUCE_SEGMENTED && it.width > 0) ? 1 : 0;
     m->max_item_sessions = 0;
     for (const Item& it : items) m->max_item_sessions = std::max<int64_t>(m->max_item_sessions, it.s1 - it.s0);
     int max_width = 0;
     for (const Item& it : items) max_width = std::max(max_width, (int)it.width);
     m->lds_bytes = m->reduce_mode == MC_REDUCE_SEGMENTED ? (size_t)max_width * T * 12 : 0;
-
-    // Reorder the sessions of each item by last-click rank (wavefront-uniform
-    // chain lengths); the per-entry sums do not depend on the order.
-    if (m->sort_by_lcr) {
-        for (const Item& it : items) {
-            std::stable_sort(order.begin() + it.s0, order.begin() + it.s1,
-                             [&](int64_t a, int64_t b) { return lcr[a] < lcr[b]; });
-        }
-    }
 
     // ---- device layout: rank-major pidx, click masks, lengths ---------------
     std::vector<int32_t> h_pidx((size_t)n * kMaxSerp);
@@ -1377,7 +1504,7 @@ mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t
         default: hipLaunchKernelGGL(eval_kernel<kCCM>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
     }
     chk(hipGetLastError());
-    hipLaunchKernelGGL(eval_reduce_kernel, dim3(1), dim3(64), 0, m->stream, d_part, (int)nb, d_out);
+    hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, d_part, (int)nb, d_out);
     chk(hipGetLastError());
     double out[21];
     chk(hipMemcpyAsync(out, d_out, 21 * 8, hipMemcpyDeviceToHost, m->stream));

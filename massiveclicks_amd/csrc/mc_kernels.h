@@ -34,13 +34,14 @@ constexpr double kFixInv = 1.0 / 4294967296.0;
 enum { kPBM = 0, kCCM = 1, kDBN = 2, kUBM = 3 };
 
 // Work item flags.
-enum { kItemInterior = 1 };
+enum { kItemInterior = 1, kItemDirect = 2 };
 
 struct Item {
     int32_t s0, s1;     // session range [s0, s1) in device order
     int32_t base;       // first qd index of the accumulator window
     int32_t width;      // window width; 0 = no window (global atomics)
-    int32_t flags;      // kItemInterior: every window entry is owned by this item
+    int32_t flags;      // kItemInterior: every window entry is owned by this item;
+                        // kItemDirect: every session owns its entries alone
     int32_t pad;
 };
 
@@ -53,7 +54,7 @@ struct TableArgs {
 };
 
 struct KArgs {
-    const int32_t* pidx;      // [kL][N] rank-major
+    const int32_t* pidx;      // [kL][N] rank-major (N * 40 bytes < 2^32)
     const uint16_t* mask;     // [N] click bits (bit r = rank r)
     const uint8_t* len;       // [N] ranks shown, or nullptr
     int64_t N;
@@ -123,26 +124,41 @@ __device__ __forceinline__ int last_click(uint32_t m) {
 // --------------------------------------------------------------------------
 // Emitters: where a per-qd expected count goes.
 // --------------------------------------------------------------------------
+// add(t, r, p, x): expected count x of per-qd table t for the entry p shown
+// at rank r of the current session.
 template <int T>
 struct EmitLds {            // fixed-point LDS window, relative indices
     unsigned long long* acc;   // [width][T]
-    __device__ __forceinline__ void add(int t, int p, float x) {
+    __device__ __forceinline__ void add(int t, int, int p, float x) {
         atomicAdd(&acc[p * T + t], (unsigned long long)to_fix(x));
     }
 };
 struct EmitGlobalFix {      // fixed-point global (item without a window)
     unsigned long long* g0;
     unsigned long long* g1;
-    __device__ __forceinline__ void add(int t, int p, float x) {
+    __device__ __forceinline__ void add(int t, int, int p, float x) {
         atomicAdd(t == 0 ? &g0[p] : &g1[p], (unsigned long long)to_fix(x));
     }
 };
 struct EmitGlobalFloat {    // float atomics (reference-like mode)
     float* g0;
     float* g1;
-    __device__ __forceinline__ void add(int t, int p, float x) {
+    __device__ __forceinline__ void add(int t, int, int p, float x) {
         unsafeAtomicAdd(t == 0 ? &g0[p] : &g1[p], x);
     }
+};
+// Sessions whose entries no other session touches (a query group of one
+// session): the counts stay in registers, indexed by rank, and the lane
+// writes the new parameter values itself.
+struct EmitRegs {
+    float x[2][kL];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int r = 0; r < kL; r++) x[t][r] = 0.f;
+    }
+    __device__ __forceinline__ void add(int t, int r, int, float v) { x[t][r] += v; }
 };
 
 // --------------------------------------------------------------------------
@@ -185,7 +201,7 @@ __device__ __forceinline__ void pbm_session(const int (&pv)[kL], int L, uint32_t
             float ae = atr * ex;
             float na = (atr - ae) / (1 - ae);
             float ne = (ex - ae) / (1 - ae);
-            em.add(0, p, na);
+            em.add(0, r, p, na);
             acc[r] += ne;
         }
     }
@@ -225,7 +241,7 @@ __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t
             float ae = atr * ex;
             float na = (atr - ae) / (1 - ae);
             float ne = (ex - ae) / (1 - ae);
-            em.add(0, p, na);
+            em.add(0, r, p, na);
             ne = last_writer ? ne : 0.f;
 #pragma unroll
             for (int j = 0; j < r; j++) acc[row + j] += (slot[r] == row + j) ? ne : 0.f;
@@ -266,6 +282,9 @@ __device__ __forceinline__ float pick10(const float (&v)[kL], int i) {
 template <bool VARLEN, class Src, class Emit>
 __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             float g, float& gn, float& gd) {
+    // The chains are restructured algebra rather than the reference's
+    // operation order, so multiply-adds may fuse here.
+#pragma clang fp contract(fast)
     const int lcr = last_click(mm);
     // Pass 1, ranks 0..L-1: exam[r] (compute_exm_car) and the no-click tail
     // products V_k; only exam[r] for r > lcr and V_k for k < L - lcr are used.
@@ -296,7 +315,7 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
         if ((!VARLEN || r < L) && r > lcr) {
             const float a = src.get(0, pv[r]);
             car = a + g * (1 - a) * car;
-            em.add(0, pv[r], fdiv(a * (1 - exam[r]), 1 - exam[r] * car));
+            em.add(0, r, pv[r], fdiv(a * (1 - exam[r]), 1 - exam[r] * car));
         }
     }
     // Pass 3, ranks lcr..L-1: satisfaction at the last click, gamma factors.
@@ -315,7 +334,7 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
             else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
             if (r == lcr) {
                 const float sv = src.get(1, pv[r]);
-                em.add(1, pv[r], fdiv(sv, 1 - (1 - sv) * g * car));
+                em.add(1, r, pv[r], fdiv(sv, 1 - (1 - sv) * g * car));
                 const float phi4 = (1 - sv) * (1 - g);
                 const float phi5 = (1 - sv) * g * Tn;
                 const float inv = __builtin_amdgcn_rcpf(phi4 + phi5 + sv);
@@ -357,6 +376,7 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
 template <bool VARLEN, class Src, class Emit>
 __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             float t1, float t2, float t3, float (&ta)[6]) {
+#pragma clang fp contract(fast)
     const int lcr = last_click(mm);
     float av[kL];
 #pragma unroll
@@ -409,7 +429,7 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
         else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
         if (r == lcr) {
             float cn = car[r + 1];
-            em.add(0, pv[r], fdiv(av[r], 1 - (t2 * (1 - av[r]) + t3 * av[r]) * cn));
+            em.add(0, r, pv[r], fdiv(av[r], 1 - (t2 * (1 - av[r]) + t3 * av[r]) * cn));
             float phi4 = (1 - av[r]) * (1 - t2);
             float phi5 = (1 - av[r]) * t2 * Tn;
             float phi6 = av[r] * (1 - t3);
@@ -424,7 +444,7 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
             ta[5] += n3 + phi6 * inv;
             ex = t2 * (1 - av[r]) + t3 * av[r];
         } else {
-            em.add(0, pv[r], fdiv((1 - exam[r]) * av[r], 1 - exam[r] * car[r]));
+            em.add(0, r, pv[r], fdiv((1 - exam[r]) * av[r], 1 - exam[r] * car[r]));
             float E = ex;
             float phi0 = 1 - E;
             float phi4 = (1 - t1) * E;

@@ -85,6 +85,22 @@ def test_params_match_oracle_after_50_iterations(model, mode):
 
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+@pytest.mark.parametrize("opts", [dict(), dict(direct=True, item_sessions=512), dict(sort_by_lcr=False, block=128),
+                                  dict(item_window=64, item_sessions=256)],
+                         ids=["default", "direct", "unsorted-b128", "tiny-windows"])
+def test_layout_options_keep_parity(model, opts):
+    """Direct single-session items, unsorted sessions, other workgroup sizes
+    and tiny windows (which force global-atomic slices) give the same
+    parameters within the bar."""
+    ds, tabs, _ = workload(model)
+    with run_gpu(model, ds, **opts) as cm:
+        lay = cm.layout_info()
+        for t, (onum, oden, oval) in enumerate(tabs):
+            err = float(np.max(np.abs(cm.get_values(t) - oval)))
+            assert err < TOL_PARAM, f"{opts} {lay} table {t}: {err}"
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
 def test_params_match_reference_faithful_oracle(model):
     ds, tabs, _ = workload(model, small=True, accum_fp64=False)
     print(f"case model={model} small n_train={len(ds['train']['query'])} n_qd={ds['n_qd']}", flush=True)
