@@ -44,7 +44,7 @@ enum {
     MC_OPT_REDUCE_MODE = 0,
     /* Target sessions per workgroup work item (default 8192). */
     MC_OPT_ITEM_SESSIONS = 1,
-    /* Per-qd accumulator window per work item, in entries (default: auto). */
+    /* Per-qd accumulator window per work item, in entries (default 1280). */
     MC_OPT_ITEM_WINDOW = 2,
     /* 1 = reorder sessions inside each work item by last-click rank so that
      * wavefronts run uniform chains (default 1). */
@@ -54,7 +54,12 @@ enum {
     /* 1 = sessions that are the only session of their query (and show
      * distinct documents) update their entries directly, without a
      * reduction (default 0). */
-    MC_OPT_DIRECT = 5
+    MC_OPT_DIRECT = 5,
+    /* 1 = window items run in one persistent kernel: each resident
+     * workgroup walks a fixed stride of items and prefetches its next item
+     * (default 1; needs MC_OPT_BLOCK 256 and a window of at most 1280
+     * entries). */
+    MC_OPT_PERSISTENT = 6
 };
 enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };
 

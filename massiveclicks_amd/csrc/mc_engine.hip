@@ -294,6 +294,126 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     }
 }
 
+// Persistent form of the window path: workgroup b of G resident ones
+// processes window items b, b + G, b + 2G, ... (items are ordered largest
+// first, so the striding balances the load; a fixed assignment keeps the
+// per-lane float sums, and so the results, identical from run to run).
+// While item k is processed, the window values of the workgroup's next item
+// and its first session are already in flight (registers), and the constants
+// of item k's entries arrive during its session loop, so the round trips of
+// staging and flushing overlap with the E-step.  The shared-table counts stay
+// in registers across items and are reduced once per workgroup.  Requires
+// blockDim.x == 256 and width <= 256 * PER.
+template <int MODEL, bool VARLEN, int PER>
+__global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int n_win) {
+    constexpr int T = Shape<MODEL>::T;
+    constexpr int K = Shape<MODEL>::K;
+    constexpr int KACC = Shape<MODEL>::KACC;
+    constexpr int NT = 256;
+    extern __shared__ unsigned long long lds[];
+    __shared__ float sh[K < 64 ? 64 : K];
+    __shared__ long long red[KACC][4];
+
+    const int tid = threadIdx.x;
+    const int Wmax = A.max_width;
+    unsigned long long* lacc = lds;
+    float* lval = (float*)(lds + T * Wmax);
+    const Stream st = make_stream(A);
+    for (int k = tid; k < K; k += NT) sh[k] = A.sh_val_in[k];
+
+    int cur = blockIdx.x;
+    Item it = cur < n_win ? A.items[cur] : Item{0, 0, 0, 0, 0, 0};
+
+    // Window values of the current item, and its first session.
+    float wv[PER][T];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int i = u * NT + tid;
+#pragma unroll
+        for (int t = 0; t < T; t++) wv[u][t] = (i < it.width) ? A.tab[t].val_in[it.base + i] : 0.f;
+    }
+    uint32_t m0;
+    int L0;
+    int pv0[kL];
+    load_session<VARLEN>(st, it.s0 + tid, m0, L0, pv0);
+
+    float acc[55];
+#pragma unroll
+    for (int k = 0; k < 55; k++) acc[k] = 0.f;
+
+    while (cur < n_win) {
+        const int W = it.width;
+        // Stage the window, take the next item.
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int i = u * NT + tid;
+            if (i < W) {
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    lacc[i * T + t] = 0ull;
+                    lval[i * T + t] = wv[u][t];
+                }
+            }
+        }
+        __syncthreads();
+        const int nxt = cur + (int)gridDim.x;
+        const Item itn = nxt < n_win ? A.items[nxt] : Item{0, 0, 0, 0, 0, 0};
+        // In flight during the session loop: next window values, this
+        // window's counts.
+        int2 cnt[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int i = u * NT + tid;
+#pragma unroll
+            for (int t = 0; t < T; t++) wv[u][t] = (i < itn.width) ? A.tab[t].val_in[itn.base + i] : 0.f;
+            cnt[u] = (i < W) ? A.counts[it.base + i] : make_int2(0, 0);
+        }
+        {
+            EmitLds<T> em{lacc};
+            const SrcLds<T> lsrc{lval};
+            run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+        }
+        // First session of the next item goes out before the flush.
+        load_session<VARLEN>(st, itn.s0 + tid, m0, L0, pv0);
+        __syncthreads();
+        const bool interior = (it.flags & kItemInterior) != 0;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int i = u * NT + tid;
+            if (i >= W) continue;
+            const int p = it.base + i;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                const unsigned long long v = lacc[i * T + t];
+                if (interior) {
+                    const float2 bd = base_den<MODEL>(t, cnt[u]);
+                    const float num = fix_to_float(bd.x, v);
+                    A.tab[t].val_out[p] = clamp_value(num, bd.y);
+                    if (A.write_num) A.tab[t].num_out[p] = num;
+                } else if (v != 0ull) {
+                    atomicAdd(&A.tab[t].gacc[p], v);
+                }
+            }
+        }
+        __syncthreads();
+        cur = nxt;
+        it = itn;
+    }
+
+    // Shared tables: one reduction and one atomic per accumulator per workgroup.
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int k = 0; k < KACC; k++) {
+        long long v = wave_sum(to_fix(acc[k]));
+        if (lane == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    for (int k = tid; k < KACC; k += NT) {
+        const long long v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        if (v != 0) atomicAdd(&A.sh_acc[k], (unsigned long long)v);
+    }
+}
+
 // Merge kernel: block 0 finalises the shared tables, the other blocks
 // finalise per-qd entries accumulated in global memory (entries merged across
 // items in segmented mode, every entry in atomic mode).
@@ -534,10 +654,12 @@ struct mc_model {
     // options
     int reduce_mode = MC_REDUCE_SEGMENTED;
     int item_sessions = 8192;
-    int item_window = 0;   // 0 = auto
+    int item_window = 1280;
     int sort_by_lcr = 1;
     int block = 256;
     int direct = 0;
+    int persistent = 1;
+    int persist_grid = 0;     // resident workgroups of the persistent window kernel
     // shape
     int T = 1, K = 10, KACC = 10;
     int64_t N = 0, n_qd = 0;
@@ -552,6 +674,7 @@ struct mc_model {
     int n_win_items = 0;      // items [0, n_win_items) have an LDS window,
     int n_glob_items = 0;     // then items merged with global atomics,
     int n_dir_items = 0;      // then direct items (single-session groups)
+    int max_width = 0;
     int64_t max_item_sessions = 0;
     int32_t* d_shared_idx = nullptr;
     int64_t n_shared = 0;
@@ -663,6 +786,7 @@ KArgs make_args(mc_model* m, int cur, int write_num) {
     A.n_tables = m->T;
     A.write_num = write_num;
     A.counts = m->d_counts;
+    A.max_width = m->max_width;
     A.K = m->K;
     A.kacc = m->KACC;
     A.sh_base = m->d_sh_base;
@@ -682,12 +806,24 @@ void launch_path(mc_model* m, const KArgs& A, int first, int count, size_t lds) 
         hipLaunchKernelGGL((em_item_kernel<MODEL, false, PATH>), dim3(count), dim3(m->block), lds, m->stream, A, first);
 }
 
+// Window width covered by the persistent kernel's register staging.
+constexpr int kPersistPer = 5;
+
 template <int MODEL>
 void launch_item(mc_model* m, const KArgs& A) {
     if (m->reduce_mode == MC_REDUCE_ATOMIC) {
         launch_path<MODEL, kPathAtomic>(m, A, 0, m->n_items, 0);
     } else {
-        launch_path<MODEL, kPathLds>(m, A, 0, m->n_win_items, m->lds_bytes);
+        if (m->persistent && m->n_win_items > 0) {
+            if (m->varlen)
+                hipLaunchKernelGGL((em_window_persistent_kernel<MODEL, true, kPersistPer>), dim3(m->persist_grid),
+                                   dim3(256), m->lds_bytes, m->stream, A, m->n_win_items);
+            else
+                hipLaunchKernelGGL((em_window_persistent_kernel<MODEL, false, kPersistPer>), dim3(m->persist_grid),
+                                   dim3(256), m->lds_bytes, m->stream, A, m->n_win_items);
+        } else {
+            launch_path<MODEL, kPathLds>(m, A, 0, m->n_win_items, m->lds_bytes);
+        }
         launch_path<MODEL, kPathGlobal>(m, A, m->n_win_items, m->n_glob_items, 0);
         launch_path<MODEL, kPathDirect>(m, A, m->n_win_items + m->n_glob_items, m->n_dir_items, 0);
     }
@@ -874,6 +1010,9 @@ mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
             break;
         case MC_OPT_DIRECT:
             m->direct = value != 0;
+            break;
+        case MC_OPT_PERSISTENT:
+            m->persistent = value != 0;
             break;
         case MC_OPT_BLOCK:
             MC_REQUIRE(value == 64 || value == 128 || value == 256 || value == 512, "block must be 64/128/256/512");
@@ -1159,7 +1298,29 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     for (const Item& it : items) m->max_item_sessions = std::max<int64_t>(m->max_item_sessions, it.s1 - it.s0);
     int max_width = 0;
     for (const Item& it : items) max_width = std::max(max_width, (int)it.width);
+    m->max_width = max_width;
     m->lds_bytes = m->reduce_mode == MC_REDUCE_SEGMENTED ? (size_t)max_width * T * 12 : 0;
+
+    // Persistent window kernel: 256-thread workgroups, windows staged through
+    // registers (width <= 256 * kPersistPer), one workgroup per resident slot.
+    m->persistent = m->persistent && m->block == 256 && max_width <= 256 * kPersistPer;
+    if (m->persistent && m->n_win_items > 0) {
+        int per_cu = 0, n_cu = 0;
+        hipError_t e1 = hipSuccess;
+        switch (m->type) {
+            case MC_PBM: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kPBM, true, kPersistPer>, 256, m->lds_bytes)
+                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kPBM, false, kPersistPer>, 256, m->lds_bytes); break;
+            case MC_UBM: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kUBM, true, kPersistPer>, 256, m->lds_bytes)
+                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kUBM, false, kPersistPer>, 256, m->lds_bytes); break;
+            case MC_DBN: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kDBN, true, kPersistPer>, 256, m->lds_bytes)
+                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kDBN, false, kPersistPer>, 256, m->lds_bytes); break;
+            default: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kCCM, true, kPersistPer>, 256, m->lds_bytes)
+                                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kCCM, false, kPersistPer>, 256, m->lds_bytes); break;
+        }
+        MC_HIP(e1);
+        MC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, m->device));
+        m->persist_grid = std::max(1, std::min(m->n_win_items, per_cu * n_cu));
+    }
 
     // ---- device layout: rank-major pidx, click masks, lengths ---------------
     std::vector<int32_t> h_pidx((size_t)n * kMaxSerp);

@@ -65,6 +65,7 @@ struct KArgs {
     const int2* counts;       // per qd entry: (occurrences, clicked occurrences) in training
     const float* sh_val_in;   // shared table values of theta_t
     unsigned long long* sh_acc;   // [kacc] fixed-point accumulators of the shared tables
+    int32_t max_width;            // largest item window (LDS sizing)
     // merge kernel
     const int32_t* shared_idx;   // entries merged across items
     int64_t n_shared;

@@ -20,7 +20,7 @@ PBM, CCM, DBN, UBM = 0, 1, 2, 3          # main.cpp:178-183
 MODEL_NAMES = {PBM: "PBM", CCM: "CCM", DBN: "DBN", UBM: "UBM"}
 MAX_SERP = 10
 
-OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK, OPT_DIRECT = range(6)
+OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK, OPT_DIRECT, OPT_PERSISTENT = range(7)
 REDUCE_SEGMENTED, REDUCE_ATOMIC = 0, 1
 
 
@@ -85,7 +85,8 @@ class ClickModel:
 
     def __init__(self, model_type: int, device: int = 0, reduce_mode: int = REDUCE_SEGMENTED,
                  item_sessions: int | None = None, item_window: int | None = None,
-                 sort_by_lcr: bool = True, block: int | None = None, direct: bool = False):
+                 sort_by_lcr: bool = True, block: int | None = None, direct: bool = False,
+                 persistent: bool = True):
         self.model_type = int(model_type)
         self._h = C.c_void_p()
         L = _lib.lib()
@@ -93,6 +94,7 @@ class ClickModel:
         check(L.mc_model_set_option(self._h, OPT_REDUCE_MODE, int(reduce_mode)))
         check(L.mc_model_set_option(self._h, OPT_SORT_BY_LCR, int(bool(sort_by_lcr))))
         check(L.mc_model_set_option(self._h, OPT_DIRECT, int(bool(direct))))
+        check(L.mc_model_set_option(self._h, OPT_PERSISTENT, int(bool(persistent))))
         if item_sessions is not None:
             check(L.mc_model_set_option(self._h, OPT_ITEM_SESSIONS, int(item_sessions)))
         if item_window is not None:
