@@ -105,9 +105,13 @@ __device__ __forceinline__ Stream make_stream(const KArgs& A) {
 
 // Loads one session's click bits, length and query-document indices
 // (rank-major layout: rank r of session s at pidx[r * N + s]).
+// Lanes past the end of their item (s >= s_end) re-read the item's last
+// session, whose cache lines are already fetched: no branch, no extra DRAM
+// traffic, and the values are never used.  s_end <= 0 (no item) reads past
+// the buffer, which returns zeros.
 template <bool VARLEN>
-__device__ __forceinline__ void load_session(const Stream& st, int s, uint32_t& m, int& L, int (&pv)[kL]) {
-    const uint32_t o = (uint32_t)s;
+__device__ __forceinline__ void load_session(const Stream& st, int s, int s_end, uint32_t& m, int& L, int (&pv)[kL]) {
+    const uint32_t o = (uint32_t)min(s, s_end - 1);
     m = __builtin_amdgcn_raw_buffer_load_b16(st.mask, o * 2u, 0, 0);
     L = VARLEN ? (int)__builtin_amdgcn_raw_buffer_load_b8(st.len, o, 0, 0) : kL;
 #pragma unroll
@@ -125,7 +129,7 @@ __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const
         uint32_t m2;
         int L2;
         int pv2[kL];
-        load_session<VARLEN>(st, s + nthr, m2, L2, pv2);
+        load_session<VARLEN>(st, s + nthr, it.s1, m2, L2, pv2);
         run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
         m = m2;
         L = L2;
@@ -146,7 +150,7 @@ __device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st
         uint32_t m2;
         int L2;
         int pv2[kL];
-        load_session<VARLEN>(st, s + nthr, m2, L2, pv2);
+        load_session<VARLEN>(st, s + nthr, it.s1, m2, L2, pv2);
         EmitRegs em;
         em.clear();
         run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
@@ -214,7 +218,7 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     uint32_t m0;
     int L0;
     int pv0[kL];
-    load_session<VARLEN>(st, it.s0 + tid, m0, L0, pv0);
+    load_session<VARLEN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
     for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
     if (win) {
         for (int i = tid; i < W; i += nthr) {
@@ -335,7 +339,7 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
     uint32_t m0;
     int L0;
     int pv0[kL];
-    load_session<VARLEN>(st, it.s0 + tid, m0, L0, pv0);
+    load_session<VARLEN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
 
     float acc[55];
 #pragma unroll
@@ -374,7 +378,7 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
             run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
         }
         // First session of the next item goes out before the flush.
-        load_session<VARLEN>(st, itn.s0 + tid, m0, L0, pv0);
+        load_session<VARLEN>(st, itn.s0 + tid, itn.s1, m0, L0, pv0);
         __syncthreads();
         const bool interior = (it.flags & kItemInterior) != 0;
 #pragma unroll
