@@ -9,9 +9,9 @@ Generative stories (one query session, ranks 0..L-1):
   PBM/UBM  C_r = A_r * E_r with A_r ~ Bern(a_r), E_r ~ Bern(e_r) independent.
   DBN      E_0 = 1; C_r = E_r A_r; after a click S_r ~ Bern(s_r) and
            E_{r+1} = (1 - S_r) G_r, otherwise E_{r+1} = E_r G_r, G_r ~ Bern(g).
-  CCM      E_0 = 1; C_r = E_r A_r; after a click E_{r+1} = H_r with
-           H_r ~ Bern(t2 (1 - a_r) + t3 a_r), otherwise E_{r+1} = E_r G_r with
-           G_r ~ Bern(t1).
+  CCM      E_0 = 1; C_r = E_r A_r; S_r ~ Bern(a_r) (satisfaction, used only
+           after a click); after a click E_{r+1} = G_r with G_r ~ Bern(t3) if
+           S_r else Bern(t2), otherwise E_{r+1} = E_r G_r with G_r ~ Bern(t1).
 """
 import itertools
 
@@ -43,10 +43,8 @@ def cascade_joint(model, a, s=None, g=None, taus=None):
             E[:, r + 1] = np.where(C[:, r] == 1, (1 - Sx[:, r]) * G[:, r], E[:, r] * G[:, r])
         else:
             t1, t2, t3 = taus
-            # the S bits are unused by the CCM: weight them 1/2 each so the
-            # enumeration stays a probability distribution
-            p *= 0.5
-            h = t2 * (1 - a[r]) + t3 * a[r]
+            p *= np.where(Sx[:, r] == 1, a[r], 1 - a[r])
+            h = np.where(Sx[:, r] == 1, t3, t2)
             cont = np.where(C[:, r] == 1, np.where(G[:, r] == 1, h, 1 - h), np.where(G[:, r] == 1, t1, 1 - t1))
             p *= cont
             E[:, r + 1] = np.where(C[:, r] == 1, G[:, r], E[:, r] * G[:, r])
@@ -116,3 +114,52 @@ def pbm_posteriors(a, e):
                 num_a += pr * A
                 num_e += pr * E
     return num_a / z, num_e / z
+
+
+def _factor_weights(C, A, E, S, p, clicks, r):
+    """w[x, y, z] = P(E_r = x, S'_r = y, E_{r+1} = z, C = clicks, A_r = c_r)
+    with S'_r = S_r C_r (satisfaction exists only after a click)."""
+    c = np.asarray(clicks)
+    m = np.all(C == c[None, :C.shape[1]], axis=1) & (A[:, r] == c[r])
+    Sp = S[:, r] * C[:, r]
+    w = np.zeros((2, 2, 2))
+    for x in (0, 1):
+        for y in (0, 1):
+            for z in (0, 1):
+                w[x, y, z] = p[m & (E[:, r] == x) & (Sp == y) & (E[:, r + 1] == z)].sum()
+    return w
+
+
+def factor_posteriors(model, clicks, **kw):
+    """Per rank r of the whole list: the posterior over (E_r, S'_r, E_{r+1})
+    given every click and A_r = c_r, as an array [L, 2, 2, 2]."""
+    C, A, E, S, p = cascade_joint(model, **kw)
+    out = []
+    for r in range(len(clicks)):
+        w = _factor_weights(C, A, E, S, p, clicks, r)
+        out.append(w / w.sum())
+    return np.array(out)
+
+
+def factor_posteriors_fresh_tail(model, clicks, a, s=None, g=None, taus=None):
+    """As factor_posteriors, except that the likelihood of the clicks after
+    rank r given E_{r+1} = 1 is that of a fresh list started at rank 0: the
+    clicks of ranks r+1 .. L-1 scored with the parameters of ranks
+    0 .. L-r-2.  Ranks 0..r are enumerated with their own parameters."""
+    L = len(clicks)
+    c = np.asarray(clicks)
+    out = []
+    for r in range(L):
+        kw = dict(a=a[:r + 1], s=None if s is None else s[:r + 1], g=g, taus=taus)
+        C, A, E, S, p = cascade_joint(model, **kw)
+        w = _factor_weights(C, A, E, S, p, c[:r + 1], r)
+        if r + 1 < L:
+            n = L - r - 1
+            kt = dict(a=a[:n], s=None if s is None else s[:n], g=g, taus=taus)
+            Ct, _, _, _, pt = cascade_joint(model, **kt)
+            t1 = pattern_prob(Ct, pt, c[r + 1:])
+            t0 = 0.0 if c[r + 1:].any() else 1.0
+            w[:, :, 1] *= t1
+            w[:, :, 0] *= t0
+        out.append(w / w.sum())
+    return np.array(out)

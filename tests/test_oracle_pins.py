@@ -2,19 +2,26 @@
 facts the mathematics fixes (the reference has no tests or golden vectors of
 its own).  Each test names the oracle function it pins.
 
-Left unpinned beyond the line-by-line restatement: the DBN gamma and CCM tau
-factor updates (dbn.cu:436-467, ccm.cu:387-467 with factor.cu:72-142,
-206-274).  They are not a textbook posterior: the unexamined branch of the
-factor is weighted by (1 - a), the tail chains mix parameters of rank j with
-clicks of rank start + j, and "no click" is encoded as last-click rank 10,
-which masks the z = 0 branches.  No closed form reproduces them.
+The DBN gamma and CCM tau factor updates (dbn.cu:436-467, ccm.cu:387-467
+with factor.cu:72-142, 206-274) are not the textbook posterior, but each of
+their three departures is a fact the enumeration can state:
+  * the no-click branch multiplies every (x, y, z) by (1 - a), so the
+    factor is the posterior over (E_r, S_r, E_{r+1}) jointly with A_r = c_r;
+  * the tail chains score the clicks after rank r with the parameters of
+    ranks 0, 1, ... (a fresh list), not of ranks r+1, r+2, ...;
+  * a session without clicks has last-click rank 10, which masks every
+    z = 0 branch, so each rank contributes exactly (1, 1).
+The tests below pin the oracle against enumeration under exactly these
+statements, and against the plain posterior when all ranks share their
+parameters (where the fresh-list tail is the true tail).
 """
 import numpy as np
 import pytest
 
 import oracle
-from bruteforce import (atr_posterior_no_later_click, conditional_click_probs, marginal_click_probs,
-                        pbm_posteriors, sat_posterior_last_click, ubm_marginal_click_probs)
+from bruteforce import (atr_posterior_no_later_click, conditional_click_probs, factor_posteriors,
+                        factor_posteriors_fresh_tail, marginal_click_probs, pbm_posteriors,
+                        sat_posterior_last_click, ubm_marginal_click_probs)
 
 PBM, CCM, DBN, UBM = 0, 1, 2, 3
 
@@ -209,6 +216,104 @@ def test_dbn_satisfaction_at_last_click_is_posterior(L):
             if r == lcr:
                 continue
             assert (sn[r], sd[r]) == ((0, 1) if clicks[r] else (1, 2))
+
+
+def _shared_factor_tmp(model, L, clicks, a, s, g, taus):
+    """Runs one E-step of the oracle on one session; returns the session's
+    gamma (DBN) or tau1..tau3 (CCM) temporaries as [(num, den), ...]."""
+    if model == DBN:
+        om, _, _ = one_session_model(DBN, L, clicks, {0: a, 1: s, 2: [g]})
+        om.estep()
+        n, d = om.get_tmp(2)
+        return [(n[0], d[0])]
+    om, _, _ = one_session_model(CCM, L, clicks, {0: a, 1: taus})
+    om.estep()
+    n, d = om.get_tmp(1)
+    return [(n[k], d[k]) for k in range(3)]
+
+
+def _expected_from_weights(model, clicks, w):
+    """Sums the factor posteriors as compute_gamma / compute_taus do."""
+    if model == DBN:
+        num = sum(w[r][1, 0, 1] for r in range(len(clicks)))
+        den = sum(w[r][1, 0, 0] + w[r][1, 0, 1] for r in range(len(clicks)))
+        return [(num, den)]
+    t = np.zeros((3, 2))
+    for r, c in enumerate(clicks):
+        if c == 0:
+            t[0] += (w[r][1, :, 1].sum(), w[r][1].sum())
+        else:
+            t[1] += (w[r][1, 0, 1], w[r][1, 0].sum())
+            t[2] += (w[r][1, 1, 1], w[r][1, 1].sum())
+    return [tuple(x) for x in t]
+
+
+def _draw(rng, L, uniform):
+    k = 1 if uniform else L
+    a = np.resize(rng.uniform(0.05, 0.9, k), L).astype(np.float32)
+    s = np.resize(rng.uniform(0.05, 0.9, k), L).astype(np.float32)
+    g = np.float32(rng.uniform(0.3, 0.95))
+    taus = rng.uniform(0.2, 0.95, 3).astype(np.float32)
+    return a, s, g, taus
+
+
+def _kw(model, a, s, g, taus):
+    if model == DBN:
+        return dict(a=a.astype(np.float64), s=s.astype(np.float64), g=float(g))
+    return dict(a=a.astype(np.float64), taus=taus.astype(np.float64))
+
+
+@pytest.mark.parametrize("model", [DBN, CCM], ids=["DBN", "CCM"])
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5])
+def test_shared_factor_update_is_posterior_when_ranks_share_parameters(model, L):
+    """compute_gamma (dbn.cu:436-467) / compute_taus (ccm.cu:387-467) with
+    DBNFactor / CCMFactor (factor.cu:206-274, 72-142): with one attractiveness
+    (and satisfaction) value at every rank, the update summed over ranks is
+    the posterior of (E_r, S_r, E_{r+1}) given the clicks and A_r = c_r, over
+    every click pattern with at least one click."""
+    name = "dbn" if model == DBN else "ccm"
+    rng = np.random.default_rng(31 * model + L)
+    for pattern in range(1, 2 ** L):
+        clicks = np.array([(pattern >> r) & 1 for r in range(L)])
+        a, s, g, taus = _draw(rng, L, uniform=True)
+        got = _shared_factor_tmp(model, L, clicks, a, s, g, taus)
+        want = _expected_from_weights(model, clicks, factor_posteriors(name, clicks, **_kw(model, a, s, g, taus)))
+        for (gn, gd), (wn, wd) in zip(got, want):
+            assert gn == pytest.approx(wn, rel=3e-5, abs=3e-6), (pattern, got, want)
+            assert gd == pytest.approx(wd, rel=3e-5, abs=3e-6), (pattern, got, want)
+
+
+@pytest.mark.parametrize("model", [DBN, CCM], ids=["DBN", "CCM"])
+@pytest.mark.parametrize("L", [2, 3, 4, 5])
+def test_shared_factor_update_scores_the_tail_as_a_fresh_list(model, L):
+    """The same update with a different value at every rank: the clicks
+    after rank r are scored with the parameters of ranks 0, 1, ...
+    (get_tail_clicks, dbn.cu:390-423, ccm.cu:338-374).  An oracle that used
+    the parameters of ranks r+1, r+2, ... fails this test."""
+    name = "dbn" if model == DBN else "ccm"
+    rng = np.random.default_rng(57 * model + L)
+    for pattern in range(1, 2 ** L):
+        clicks = np.array([(pattern >> r) & 1 for r in range(L)])
+        a, s, g, taus = _draw(rng, L, uniform=False)
+        got = _shared_factor_tmp(model, L, clicks, a, s, g, taus)
+        w = factor_posteriors_fresh_tail(name, clicks, **_kw(model, a, s, g, taus))
+        want = _expected_from_weights(model, clicks, w)
+        for (gn, gd), (wn, wd) in zip(got, want):
+            assert gn == pytest.approx(wn, rel=3e-5, abs=3e-6), (pattern, got, want)
+            assert gd == pytest.approx(wd, rel=3e-5, abs=3e-6), (pattern, got, want)
+
+
+@pytest.mark.parametrize("model", [DBN, CCM], ids=["DBN", "CCM"])
+@pytest.mark.parametrize("L", [1, 4, 10])
+def test_shared_factor_update_of_a_session_without_clicks(model, L):
+    """No click: last-click rank 10 masks every z = 0 branch, so each rank
+    adds exactly (1, 1) to gamma / tau1 and nothing to tau2, tau3."""
+    rng = np.random.default_rng(L)
+    a, s, g, taus = _draw(rng, L, uniform=False)
+    got = _shared_factor_tmp(model, L, np.zeros(L, int), a, s, g, taus)
+    want = [(L, L)] if model == DBN else [(L, L), (0, 0), (0, 0)]
+    for (gn, gd), (wn, wd) in zip(got, want):
+        assert gn == pytest.approx(wn, rel=1e-6) and gd == pytest.approx(wd, rel=1e-6)
 
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
