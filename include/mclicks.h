@@ -81,6 +81,15 @@ mc_status mc_dataset_parse(const char* path, int max_sessions, float test_share,
 mc_status mc_dataset_from_sessions(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
                                    float test_share, mc_dataset** out);
 
+/* Same for result lists of 1..10 ranks (len[n]); ranks r >= len[s] hold no
+ * result: their docs are ignored, their clicks must be 0, they get no
+ * query-document index (pidx -1) and do not count as seen documents when
+ * test sessions are filtered.  The reference has fixed 10-rank lists
+ * (utils/definitions.h:50, dataset.cpp:266); with every len = 10 this equals
+ * mc_dataset_from_sessions. */
+mc_status mc_dataset_from_sessions_len(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
+                                       const uint8_t* len, float test_share, mc_dataset** out);
+
 mc_status mc_dataset_sizes(const mc_dataset* ds, int64_t* n_parsed, int64_t* n_train, int64_t* n_test,
                            int64_t* n_qd);
 
@@ -88,6 +97,10 @@ mc_status mc_dataset_sizes(const mc_dataset* ds, int64_t* n_parsed, int64_t* n_t
  * Arrays sized n, n*10, n*10, n*10.  pidx is -1 for unseen test pairs. */
 mc_status mc_dataset_export(const mc_dataset* ds, int which, int32_t* query, int32_t* docs, uint8_t* clicks,
                             int32_t* pidx);
+
+/* Ranks shown per session of the train (0) or test (1) set, len[n]; all 10
+ * for datasets prepared without lengths. */
+mc_status mc_dataset_export_len(const mc_dataset* ds, int which, uint8_t* len);
 
 void mc_dataset_free(mc_dataset* ds);
 
@@ -100,6 +113,9 @@ typedef struct mc_model mc_model;
 mc_status mc_model_create(int model_type, int device, mc_model** out);
 
 mc_status mc_model_set_option(mc_model* m, int option, int64_t value);
+
+/* The model id given to mc_model_create (config.model_type, main.cpp:230). */
+mc_status mc_get_model_type(const mc_model* m, int* model_type);
 
 /* ClickModel_Hst::init_parameters (base.cuh:39, common.cu:31-61) together
  * with the one-time dataset upload (parallel_em.cu:248-253).
@@ -139,7 +155,9 @@ mc_status mc_synchronize(mc_model* m);
 /* get_parameter_information (base.cuh:33): table t's name, whether it is
  * public (rank-indexed, shared by all queries) and its length.  Tables are
  * numbered per-qd first: attractiveness (all), satisfaction (DBN), then the
- * public one: examination (PBM 10, UBM 55), tau (CCM 3), gamma (DBN 1). */
+ * public one: examination (PBM 10, UBM 55), continuation (CCM tau1..tau3,
+ * DBN gamma).  Names as in get_parameter_information (pbm.cu:124-125,
+ * ubm.cu:118-119, ccm.cu:118-119, dbn.cu:135-136). */
 mc_status mc_get_n_tables(const mc_model* m, int* n_tables);
 mc_status mc_get_table_info(const mc_model* m, int t, const char** name, int* is_public, int64_t* size);
 
@@ -174,6 +192,35 @@ mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, s
  * entries merged across items, and the largest item in sessions. */
 mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
                              int64_t* max_item_sessions);
+
+/* ---------------------------------------------------------------------------
+ * Parameter output and checkpoints.
+ * ------------------------------------------------------------------------- */
+
+/* Communicate::output_parameters (parallel_em/communicator.cpp:415-516) for
+ * one worker: one file <prefix>_<name>.csv per table (names as in
+ * mc_get_table_info).  Public table: a "rank, <name>" header, then one
+ * "<rank>, <value>" line per entry (ostream default formatting, 6
+ * significant digits).  Per-qd table: one "<query>, <document>, <value>"
+ * line per rank of every training session, in train order (std::to_string,
+ * 6 decimals); the reference writes that file's header and then the data
+ * over it from offset 0, and the resulting bytes (the data, then whatever of
+ * the header extends past it) are reproduced.  value = Param::value(). */
+mc_status mc_output_parameters(mc_model* m, const mc_dataset* ds, const char* prefix);
+
+/* The same files from host tables: num[t], den[t] for each table t of
+ * model_type in mc_get_table_info order (per-qd tables of ds's n_qd
+ * entries, then the public table).  Needs no device. */
+mc_status mc_write_parameters_csv(const mc_dataset* ds, const char* prefix, int model_type,
+                                  const float* const* num, const float* const* den);
+
+/* Binary checkpoint of every table (numerators and denominators) for
+ * resuming training: continuing from a loaded checkpoint gives the same
+ * iterates as an uninterrupted run.  No counterpart in the reference, whose
+ * only output is the CSV above (values only).  Load requires a model of the
+ * same type initialised with the same n_qd. */
+mc_status mc_save_parameters(mc_model* m, const char* path);
+mc_status mc_load_parameters(mc_model* m, const char* path);
 
 /* destroy_parameters (base.cuh:44) and the model object. */
 void mc_model_destroy(mc_model* m);

@@ -19,7 +19,9 @@ vp = C.c_void_p
 _SIGS = {
     "mc_dataset_parse": (C.c_int, [C.c_char_p, C.c_int, C.c_float, C.POINTER(vp)]),
     "mc_dataset_from_sessions": (C.c_int, [C.c_int64, i32p, i32p, u8p, C.c_float, C.POINTER(vp)]),
+    "mc_dataset_from_sessions_len": (C.c_int, [C.c_int64, i32p, i32p, u8p, u8p, C.c_float, C.POINTER(vp)]),
     "mc_dataset_sizes": (C.c_int, [vp, i64p, i64p, i64p, i64p]),
+    "mc_dataset_export_len": (C.c_int, [vp, C.c_int, u8p]),
     "mc_dataset_export": (C.c_int, [vp, C.c_int, i32p, i32p, u8p, i32p]),
     "mc_dataset_free": (None, [vp]),
     "mc_model_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(vp)]),
@@ -41,6 +43,11 @@ _SIGS = {
     "mc_get_memory_usage": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
     "mc_compute_memory_footprint": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_size_t)]),
     "mc_get_layout_info": (C.c_int, [vp, i64p, i64p, i64p]),
+    "mc_get_model_type": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "mc_output_parameters": (C.c_int, [vp, vp, C.c_char_p]),
+    "mc_write_parameters_csv": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(f32p), C.POINTER(f32p)]),
+    "mc_save_parameters": (C.c_int, [vp, C.c_char_p]),
+    "mc_load_parameters": (C.c_int, [vp, C.c_char_p]),
     "mc_model_destroy": (None, [vp]),
     "mc_last_error": (C.c_char_p, []),
 }

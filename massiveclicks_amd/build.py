@@ -1,4 +1,5 @@
-"""Builds the HIP engine (libmclicks.so) for gfx950 with hipcc.
+"""Builds the HIP engine (libmclicks.so) for gfx950 and its command-line
+driver (mclicks) with hipcc.
 
 The library is built in-tree (massiveclicks_amd/lib/) so that it travels with
 the source tree; it is rebuilt on every call to :func:`build`.
@@ -12,11 +13,12 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmclicks.so")
+CLI = os.path.join(LIBDIR, "mclicks")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mc_engine.hip", "mc_dataset.cpp"]
+SOURCES = ["mc_engine.hip", "mc_dataset.cpp", "mc_io.cpp"]
 
 # -ffp-contract=off keeps the E-step arithmetic in the reference's operation
 # order; packed FP32 instructions are disabled on the device.
@@ -32,17 +34,25 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+def _run(cmd, what, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libmclicks.so")
+        raise RuntimeError(f"hipcc failed building {what}")
+
+
+def build(verbose: bool = False) -> str:
+    """Builds lib/libmclicks.so and the command-line driver lib/mclicks."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES], "libmclicks.so", verbose)
     os.replace(tmp, LIB)
+    tmp = CLI + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "mc_cli.cpp"),
+          "-L", LIBDIR, "-lmclicks", "-Wl,-rpath,$ORIGIN", "-o", tmp], "mclicks", verbose)
+    os.replace(tmp, CLI)
     return LIB
 
 

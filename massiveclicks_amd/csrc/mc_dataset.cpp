@@ -159,11 +159,15 @@ private:
 }  // namespace
 
 // Split, sort and index `all` (log order) into ds.  Returns false with an
-// error message when the split is empty.
+// error message when the split is empty.  len_all (optional, log order)
+// gives the ranks each session shows; ranks >= len hold no result: they are
+// not indexed (pidx -1) and do not count as seen documents of a test session.
 bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const std::vector<int32_t>& d_all,
-                     const std::vector<uint8_t>& c_all, float test_share, std::string* err) {
+                     const std::vector<uint8_t>& c_all, float test_share, std::string* err,
+                     const std::vector<uint8_t>* len_all) {
     const int n = (int)q_all.size();
     ds.n_parsed = n;
+    auto shown = [&](int src, int r) { return !len_all || r < (*len_all)[(size_t)src]; };
     if (n < 1) { *err = "no complete session"; return false; }
     const int n_train = static_cast<int>(n * (1 - test_share) - 0.5) + 1;   // dataset.cpp:390
     if (n_train < 1 || n_train > n) {
@@ -182,12 +186,17 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
     ds.train_docs.resize((size_t)n_train * kMaxSerp);
     ds.train_clicks.resize((size_t)n_train * kMaxSerp);
     ds.train_pidx.resize((size_t)n_train * kMaxSerp);
+    ds.train_len.clear();
+    ds.test_len.clear();
+    if (len_all) ds.train_len.resize((size_t)n_train);
     for (int i = 0; i < n_train; i++) {
         int src = keys[i].pos;
         ds.train_query[i] = q_all[src];
         std::memcpy(&ds.train_docs[(size_t)i * kMaxSerp], &d_all[(size_t)src * kMaxSerp], kMaxSerp * 4);
         std::memcpy(&ds.train_clicks[(size_t)i * kMaxSerp], &c_all[(size_t)src * kMaxSerp], kMaxSerp);
+        if (len_all) ds.train_len[i] = (*len_all)[(size_t)src];
     }
+    auto train_shown = [&](int i, int r) { return !len_all || r < ds.train_len[(size_t)i]; };
 
     // First-occurrence indices; equal queries are contiguous after the sort,
     // so each query group is numbered with a group-local map.
@@ -199,6 +208,10 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
         gmap.reset((size_t)std::min(e - s, 64) * kMaxSerp);
         for (int i = s; i < e; i++) {
             for (int r = 0; r < kMaxSerp; r++) {
+                if (!train_shown(i, r)) {
+                    ds.train_pidx[(size_t)i * kMaxSerp + r] = -1;
+                    continue;
+                }
                 bool ins;
                 int idx = gmap.find_or_insert(ds.train_docs[(size_t)i * kMaxSerp + r], next, &ins);
                 if (ins) next++;
@@ -221,8 +234,9 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
         train_queries.push_back(ds.train_query[s]);
         for (int i = s; i < e; i++)
             for (int r = 0; r < kMaxSerp; r++)
-                pairs.insert(ds.train_query[i], ds.train_docs[(size_t)i * kMaxSerp + r],
-                             ds.train_pidx[(size_t)i * kMaxSerp + r]);
+                if (train_shown(i, r))
+                    pairs.insert(ds.train_query[i], ds.train_docs[(size_t)i * kMaxSerp + r],
+                                 ds.train_pidx[(size_t)i * kMaxSerp + r]);
         s = e;
     }
     // train_queries is ascending (sorted train set).
@@ -236,10 +250,11 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
         int32_t pid[kMaxSerp];
         bool found = false;
         for (int r = 0; r < kMaxSerp; r++) {
-            pid[r] = pairs.find(q, d_all[(size_t)i * kMaxSerp + r]);
+            pid[r] = shown(i, r) ? pairs.find(q, d_all[(size_t)i * kMaxSerp + r]) : -1;
             if (pid[r] != -1) found = true;
         }
         if (!found) continue;
+        if (len_all) ds.test_len.push_back((*len_all)[(size_t)i]);
         ds.test_query.push_back(q);
         ds.test_docs.insert(ds.test_docs.end(), &d_all[(size_t)i * kMaxSerp], &d_all[(size_t)i * kMaxSerp] + kMaxSerp);
         ds.test_clicks.insert(ds.test_clicks.end(), &c_all[(size_t)i * kMaxSerp],

@@ -647,10 +647,6 @@ __global__ void fill_kernel(float* p, float v, int64_t n) {
 // ===========================================================================
 using namespace mc;
 
-struct mc_dataset {
-    HostDataset d;
-};
-
 struct mc_model {
     int type = 0;
     int device = 0;
@@ -711,8 +707,8 @@ const char* table_name(int type, int t) {
     switch (type) {
         case MC_PBM: return t == 0 ? "attractiveness" : "examination";
         case MC_UBM: return t == 0 ? "attractiveness" : "examination";
-        case MC_CCM: return t == 0 ? "attractiveness" : "tau";
-        case MC_DBN: return t == 0 ? "attractiveness" : (t == 1 ? "satisfaction" : "gamma");
+        case MC_CCM: return t == 0 ? "attractiveness" : "continuation";
+        case MC_DBN: return t == 0 ? "attractiveness" : (t == 1 ? "satisfaction" : "continuation");
     }
     return "?";
 }
@@ -925,14 +921,28 @@ mc_status mc_dataset_parse(const char* path, int max_sessions, float test_share,
 
 mc_status mc_dataset_from_sessions(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
                                    float test_share, mc_dataset** out) {
+    return mc_dataset_from_sessions_len(n, query, docs, clicks, nullptr, test_share, out);
+}
+
+mc_status mc_dataset_from_sessions_len(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
+                                       const uint8_t* len, float test_share, mc_dataset** out) {
     MC_REQUIRE(out && n > 0 && query && docs && clicks, "invalid argument");
     MC_REQUIRE(n <= INT32_MAX, "too many sessions");
+    if (len) {
+        for (int64_t s = 0; s < n; s++) {
+            MC_REQUIRE(len[s] >= 1 && len[s] <= kMaxSerp, "len must be in [1, 10] (session " + std::to_string(s) + ")");
+            for (int r = len[s]; r < kMaxSerp; r++)
+                MC_REQUIRE(clicks[s * kMaxSerp + r] == 0, "click on a rank past len (session " + std::to_string(s) + ")");
+        }
+    }
     std::string err;
     try {
         std::vector<int32_t> q(query, query + n), d(docs, docs + n * kMaxSerp);
         std::vector<uint8_t> c(clicks, clicks + n * kMaxSerp);
+        std::vector<uint8_t> l;
+        if (len) l.assign(len, len + n);
         mc_dataset* ds = new mc_dataset;
-        if (!prepare_dataset(ds->d, q, d, c, test_share, &err)) {
+        if (!prepare_dataset(ds->d, q, d, c, test_share, &err, len ? &l : nullptr)) {
             delete ds;
             set_error(err);
             return MC_ERR_INVALID_ARG;
@@ -970,6 +980,16 @@ mc_status mc_dataset_export(const mc_dataset* ds, int which, int32_t* query, int
     return MC_OK;
 }
 
+mc_status mc_dataset_export_len(const mc_dataset* ds, int which, uint8_t* len) {
+    MC_REQUIRE(ds && (which == 0 || which == 1) && len, "invalid argument");
+    const HostDataset& d = ds->d;
+    const auto& L = which == 0 ? d.train_len : d.test_len;
+    const size_t n = which == 0 ? d.train_query.size() : d.test_query.size();
+    if (L.empty()) std::fill(len, len + n, (uint8_t)kMaxSerp);
+    else std::copy(L.begin(), L.end(), len);
+    return MC_OK;
+}
+
 void mc_dataset_free(mc_dataset* ds) { delete ds; }
 
 mc_status mc_model_create(int model_type, int device, mc_model** out) {
@@ -990,6 +1010,12 @@ mc_status mc_model_create(int model_type, int device, mc_model** out) {
         return MC_ERR_HIP;
     }
     *out = m;
+    return MC_OK;
+}
+
+mc_status mc_get_model_type(const mc_model* m, int* model_type) {
+    MC_REQUIRE(m && model_type, "null argument");
+    *model_type = m->type;
     return MC_OK;
 }
 
@@ -1403,7 +1429,7 @@ mc_status mc_init_from_dataset(mc_model* m, const mc_dataset* ds) {
     MC_REQUIRE(m && ds, "null argument");
     const HostDataset& d = ds->d;
     return mc_init_parameters(m, (int64_t)d.train_query.size(), d.train_query.data(), d.train_pidx.data(),
-                              d.train_clicks.data(), nullptr, d.n_qd);
+                              d.train_clicks.data(), d.train_len.empty() ? nullptr : d.train_len.data(), d.n_qd);
 }
 
 mc_status mc_process_session(mc_model* m) {

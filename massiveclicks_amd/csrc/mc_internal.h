@@ -18,16 +18,23 @@ struct HostDataset {
     std::vector<uint8_t> train_clicks;
     std::vector<int32_t> test_query, test_docs, test_pidx;
     std::vector<uint8_t> test_clicks;
+    // Ranks shown per session; empty when every session shows 10.
+    std::vector<uint8_t> train_len, test_len;
 };
 
 bool parse_raw_log(const char* path, int max_sessions, std::vector<int32_t>* q_all, std::vector<int32_t>* d_all,
                    std::vector<uint8_t>* c_all, std::string* err);
 
 bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const std::vector<int32_t>& d_all,
-                     const std::vector<uint8_t>& c_all, float test_share, std::string* err);
+                     const std::vector<uint8_t>& c_all, float test_share, std::string* err,
+                     const std::vector<uint8_t>* len_all = nullptr);
 
 void set_error(const std::string& msg);
 
 }  // namespace mc
+
+struct mc_dataset {
+    mc::HostDataset d;
+};
 
 #endif  // MC_INTERNAL_H

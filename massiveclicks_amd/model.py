@@ -40,8 +40,9 @@ class SessionSet:
 class Dataset:
     """Parsed, split, sorted and indexed click log (dataset.cpp:421-455)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, has_length=False):
         self._h = handle
+        self.has_length = has_length
         L = _lib.lib()
         n = [C.c_int64() for _ in range(4)]
         check(L.mc_dataset_sizes(self._h, *[C.byref(x) for x in n]))
@@ -56,23 +57,30 @@ class Dataset:
         return cls(h)
 
     @classmethod
-    def from_sessions(cls, query, docs, clicks, test_share: float = 0.2) -> "Dataset":
+    def from_sessions(cls, query, docs, clicks, test_share: float = 0.2, length=None) -> "Dataset":
+        """In-memory sessions in log order; `length` (ranks shown, 1..10)
+        makes ranks past each session's length absent."""
         q = np.ascontiguousarray(query, dtype=np.int32)
         d = np.ascontiguousarray(docs, dtype=np.int32)
         c = np.ascontiguousarray(clicks, dtype=np.uint8)
+        ln = None if length is None else np.ascontiguousarray(length, dtype=np.uint8)
         h = C.c_void_p()
-        check(_lib.lib().mc_dataset_from_sessions(len(q), ptr(q, i32p), ptr(d, i32p), ptr(c, u8p),
-                                                   float(test_share), C.byref(h)))
-        return cls(h)
+        check(_lib.lib().mc_dataset_from_sessions_len(len(q), ptr(q, i32p), ptr(d, i32p), ptr(c, u8p),
+                                                       ptr(ln, u8p), float(test_share), C.byref(h)))
+        return cls(h, has_length=ln is not None)
 
     def _export(self, which, n):
         q = np.empty(n, np.int32)
         d = np.empty((n, MAX_SERP), np.int32)
         c = np.empty((n, MAX_SERP), np.uint8)
         p = np.empty((n, MAX_SERP), np.int32)
+        ln = None
         if n:
             check(_lib.lib().mc_dataset_export(self._h, which, ptr(q, i32p), ptr(d, i32p), ptr(c, u8p), ptr(p, i32p)))
-        return SessionSet(q, d, c, p)
+            if self.has_length:
+                ln = np.empty(n, np.uint8)
+                check(_lib.lib().mc_dataset_export_len(self._h, which, ptr(ln, u8p)))
+        return SessionSet(q, d, c, p, ln)
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -931,9 +931,9 @@ orc_model* orc_model_create(int type, int64_t n, const int32_t* query, const int
     switch (type) {   // table sizes: pbm.cu:80-96, ubm.cu:74-90, dbn.cu:76-105, ccm.cu:74-90
         case PBM: add("attractiveness", n_qd, n * kMaxSerp, 0); add("examination", kMaxSerp, n * kMaxSerp, 1); break;
         case UBM: add("attractiveness", n_qd, n * kMaxSerp, 0); add("examination", 55, n * 55, 1); break;
-        case CCM: add("attractiveness", n_qd, n * kMaxSerp, 0); add("tau", 3, n * 3, 1); break;
+        case CCM: add("attractiveness", n_qd, n * kMaxSerp, 0); add("continuation", 3, n * 3, 1); break;
         case DBN: add("attractiveness", n_qd, n * kMaxSerp, 0); add("satisfaction", n_qd, n * kMaxSerp, 0);
-                  add("gamma", 1, n, 1); break;
+                  add("continuation", 1, n, 1); break;
     }
     return h;
 }

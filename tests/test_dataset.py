@@ -134,3 +134,57 @@ def test_parse_errors_are_reported():
     with pytest.raises(MCError) as e:
         Dataset.from_sessions(np.zeros(1, np.int32), np.zeros((1, 10), np.int32), np.zeros((1, 10), np.uint8), -0.5)
     assert e.value.status == 1
+
+
+def test_lengths_of_ten_equal_the_fixed_length_preparation():
+    """mc_dataset_from_sessions_len with every len = 10 is
+    mc_dataset_from_sessions (and so the reference's preparation)."""
+    s = synth.generate(synth.SynthConfig(n_sessions=3000, n_queries=300, vocab=6000, model="dbn", seed=5))
+    a = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2)
+    b = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2, length=np.full(3000, 10, np.uint8))
+    assert a.n_qd == b.n_qd
+    for part in ("train", "test"):
+        for k in ("query", "docs", "clicks", "pidx"):
+            np.testing.assert_array_equal(getattr(getattr(a, part), k), getattr(getattr(b, part), k))
+        assert (getattr(b, part).length == 10).all()
+
+
+def test_short_result_lists_index_only_shown_ranks():
+    """Hand-worked: train = two sessions of query 7 (after the split of 3 at
+    0.2): [1, 2, 3] (len 3) and [3, 4] (len 2); ranks past len hold docs that
+    must be ignored.  Indices in first-occurrence order over shown ranks:
+    1 -> 0, 2 -> 1, 3 -> 2, 4 -> 3.  The test session shows only doc 9
+    (unseen; its rank 1 holds doc 1, which is not shown), so it is dropped."""
+    q = np.array([7, 7, 7], np.int32)
+    docs = np.full((3, 10), 1, np.int32)
+    docs[0, :3] = [1, 2, 3]
+    docs[1, :2] = [3, 4]
+    docs[2, :1] = [9]
+    clicks = np.zeros((3, 10), np.uint8)
+    clicks[1, 1] = 1
+    ln = np.array([3, 2, 1], np.uint8)
+    d = Dataset.from_sessions(q, docs, clicks, 0.2, length=ln)
+    assert (len(d.train), len(d.test), d.n_qd) == (2, 0, 4)
+    np.testing.assert_array_equal(d.train.length, [3, 2])
+    np.testing.assert_array_equal(d.train.pidx[0], [0, 1, 2] + [-1] * 7)
+    np.testing.assert_array_equal(d.train.pidx[1], [2, 3] + [-1] * 8)
+    # the same log with the test session's doc 1 at rank 0: kept, len 1
+    docs[2, 0] = 1
+    d = Dataset.from_sessions(q, docs, clicks, 0.2, length=ln)
+    assert len(d.test) == 1
+    np.testing.assert_array_equal(d.test.pidx[0], [0] + [-1] * 9)
+    np.testing.assert_array_equal(d.test.length, [1])
+
+
+def test_length_errors():
+    q = np.zeros(2, np.int32)
+    docs = np.zeros((2, 10), np.int32)
+    clicks = np.zeros((2, 10), np.uint8)
+    for bad in ([0, 10], [11, 3]):
+        with pytest.raises(MCError) as e:
+            Dataset.from_sessions(q, docs, clicks, 0.2, length=np.array(bad, np.uint8))
+        assert e.value.status == 1
+    clicks[1, 5] = 1
+    with pytest.raises(MCError) as e:
+        Dataset.from_sessions(q, docs, clicks, 0.2, length=np.array([10, 5], np.uint8))
+    assert "past len" in str(e.value)

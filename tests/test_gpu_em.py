@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import oracle
-from massiveclicks_amd import CCM, DBN, PBM, UBM, ClickModel, REDUCE_ATOMIC, REDUCE_SEGMENTED, synth
+from massiveclicks_amd import CCM, DBN, PBM, UBM, ClickModel, Dataset, REDUCE_ATOMIC, REDUCE_SEGMENTED, synth
 from massiveclicks_amd.model import SessionSet
 
 pytestmark = pytest.mark.gpu
@@ -171,3 +171,45 @@ def test_stepwise_api_equals_em_iterations(model):
         np.testing.assert_array_equal(n, ref[t][0])
         np.testing.assert_array_equal(d, ref[t][1])
     cm.close()
+
+
+_vl_cache = {}
+
+
+def varlen_workload(model):
+    """Result lists of 1..10 ranks (uniform), prepared by the engine's host
+    code; the oracle runs the same train set with the same lengths."""
+    if model not in _vl_cache:
+        name = {PBM: "pbm", UBM: "ubm", DBN: "dbn", CCM: "ccm"}[model]
+        s = synth.generate(synth.SynthConfig(n_sessions=8000, n_queries=800, vocab=30000, model=name,
+                                             variable_length=True, seed=300 + model))
+        ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2, length=s.length)
+        tr, te = ds.train, ds.test
+        om = oracle.OracleModel(model, tr.query, tr.pidx, tr.clicks, ds.n_qd, length=tr.length, accum_fp64=True)
+        om.iterate(ITERS)
+        tabs = [om.get_table(t) for t in range(om.n_tables())]
+        ev = om.evaluate(te.pidx, te.clicks, te.length)
+        _vl_cache[model] = (ds, tabs, ev)
+    return _vl_cache[model]
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+@pytest.mark.parametrize("mode", [REDUCE_SEGMENTED, REDUCE_ATOMIC], ids=["segmented", "atomic"])
+def test_variable_length_lists_match_oracle(model, mode):
+    """Uneven result lists (the batched configuration): parameters after 50
+    iterations within 1e-5, held-out LL and perplexity sums within 1e-6."""
+    ds, tabs, oev = varlen_workload(model)
+    print(f"case varlen model={model} mode={mode} n_train={ds.n_train} n_qd={ds.n_qd}", flush=True)
+    assert ds.train.length is not None and ds.train.length.min() == 1
+    with ClickModel(model, device=0, reduce_mode=mode) as cm:
+        cm.init_from_dataset(ds)
+        cm.em(ITERS)
+        for t, (onum, oden, oval) in enumerate(tabs):
+            err = float(np.max(np.abs(cm.get_values(t) - oval)))
+            assert err < TOL_PARAM, f"table {t}: max |value - oracle| = {err}"
+        ev = cm.evaluate(ds.test)
+    rel = abs(ev["llh_sum"] - oev["llh_sum_f64"]) / abs(oev["llh_sum_f64"])
+    assert rel < TOL_LL, f"LL rel diff {rel}"
+    prel = np.abs(ev["ppl_log2_sum"] - oev["ppl_log2_sum_f64"]) / np.abs(oev["ppl_log2_sum_f64"])
+    assert prel.max() < TOL_LL
+    np.testing.assert_array_equal(ev["rank_count"], oev["rank_count"])
