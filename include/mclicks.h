@@ -59,7 +59,11 @@ enum {
      * workgroup walks a fixed stride of items and prefetches its next item
      * (default 1; needs MC_OPT_BLOCK 256 and a window of at most 1280
      * entries). */
-    MC_OPT_PERSISTENT = 6
+    MC_OPT_PERSISTENT = 6,
+    /* Minibatches of the gradient path (mc_adam_steps): the query-sorted
+     * training set is cut into this many runs of whole query groups of about
+     * equal session counts (default 1 = full batch). */
+    MC_OPT_BATCHES = 7
 };
 enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };
 
@@ -192,6 +196,45 @@ mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, s
  * entries merged across items, and the largest item in sessions. */
 mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
                              int64_t* max_item_sessions);
+
+/* ---------------------------------------------------------------------------
+ * Gradient path: direct maximum-likelihood fitting beside plain EM.  No
+ * counterpart in the reference, which only runs EM.  The objective is the
+ * training log-likelihood the reference's evaluation defines
+ * (get_log_conditional_click_probs summed over ranks, e.g. dbn.cu:611-639)
+ * as a function of logits: every table value is sigmoid(logit).  Needs the
+ * segmented layout without direct items.
+ * ------------------------------------------------------------------------- */
+
+/* Logits from the current values (logit = log(v / (1 - v))), Adam moments
+ * and step counts to zero. */
+mc_status mc_grad_init(mc_model* m);
+
+/* Log-likelihood of the training sessions of minibatch `batch` (-1 = all)
+ * at the current values, and its gradient with respect to every logit:
+ * grad[t] for t < n_tables (table sizes as in mc_get_table_info; entries
+ * outside the batch are 0).  grad or any grad[t] may be NULL. */
+mc_status mc_loglik_gradient(mc_model* m, int batch, double* loglik, float* const* grad);
+
+/* Adam hyper-parameters (defaults 0.05, 0.9, 0.999, 1e-8, prior on).  With
+ * use_prior the objective adds log Beta(1 + w, 1 + w) per parameter, w = 1
+ * plus the temporaries the reference leaves at (1,2) (DBN satisfaction:
+ * unclicked occurrences; PBM/UBM examination: sessions not writing the
+ * slot), which makes EM's fixed point for those tables a stationary point. */
+mc_status mc_adam_configure(mc_model* m, float lr, float beta1, float beta2, float eps, int use_prior);
+
+/* n_steps Adam steps (ascent on the objective), one minibatch each, batches
+ * in order 0, 1, ..., MC_OPT_BATCHES - 1, 0, ...; whole epochs replay one
+ * captured graph when use_graph != 0.  Afterwards mc_get_parameters returns
+ * (value, 1) for the per-qd tables. */
+mc_status mc_adam_steps(mc_model* m, int n_steps, int use_graph);
+
+/* mc_adam_steps(use_graph = 1) timed with HIP events on the model's stream. */
+mc_status mc_adam_steps_timed(mc_model* m, int n_steps, double* ms);
+
+/* Logits of table t (after mc_grad_init); setting them sets the values. */
+mc_status mc_get_logits(mc_model* m, int t, float* logits);
+mc_status mc_set_logits(mc_model* m, int t, const float* logits);
 
 /* ---------------------------------------------------------------------------
  * Parameter output and checkpoints.

@@ -44,6 +44,13 @@ _SIGS = {
     "mc_compute_memory_footprint": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_size_t)]),
     "mc_get_layout_info": (C.c_int, [vp, i64p, i64p, i64p]),
     "mc_get_model_type": (C.c_int, [vp, C.POINTER(C.c_int)]),
+    "mc_grad_init": (C.c_int, [vp]),
+    "mc_loglik_gradient": (C.c_int, [vp, C.c_int, f64p, C.POINTER(f32p)]),
+    "mc_adam_configure": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]),
+    "mc_adam_steps": (C.c_int, [vp, C.c_int, C.c_int]),
+    "mc_adam_steps_timed": (C.c_int, [vp, C.c_int, f64p]),
+    "mc_get_logits": (C.c_int, [vp, C.c_int, f32p]),
+    "mc_set_logits": (C.c_int, [vp, C.c_int, f32p]),
     "mc_output_parameters": (C.c_int, [vp, vp, C.c_char_p]),
     "mc_write_parameters_csv": (C.c_int, [vp, C.c_char_p, C.c_int, C.POINTER(f32p), C.POINTER(f32p)]),
     "mc_save_parameters": (C.c_int, [vp, C.c_char_p]),

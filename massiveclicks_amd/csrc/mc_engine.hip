@@ -26,6 +26,7 @@
 #include "mclicks.h"
 #include "mc_internal.h"
 #include "mc_kernels.h"
+#include "mc_model.h"
 
 namespace mc {
 
@@ -34,33 +35,9 @@ thread_local std::string g_last_error;
 }
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-#define MC_HIP(call)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess) {                                                               \
-            set_error(std::string(#call) + ": " + hipGetErrorString(e_));                     \
-            return MC_ERR_HIP;                                                                \
-        }                                                                                     \
-    } while (0)
-
-#define MC_REQUIRE(cond, msg)                                                                 \
-    do {                                                                                      \
-        if (!(cond)) {                                                                        \
-            set_error(msg);                                                                   \
-            return MC_ERR_INVALID_ARG;                                                        \
-        }                                                                                     \
-    } while (0)
-
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
-
-// Number of per-qd tables, shared-table length and shared accumulators.
-template <int MODEL> struct Shape;
-template <> struct Shape<kPBM> { static constexpr int T = 1, K = 10, KACC = 10; };
-template <> struct Shape<kUBM> { static constexpr int T = 1, K = 55, KACC = 55; };
-template <> struct Shape<kDBN> { static constexpr int T = 2, K = 1, KACC = 2; };
-template <> struct Shape<kCCM> { static constexpr int T = 1, K = 3, KACC = 6; };
 
 template <int MODEL, bool VARLEN, class Src, class Emit>
 __device__ __forceinline__ void run_session(const int (&pv)[kL], int L, uint32_t m, const Src& src, Emit& em,
@@ -83,39 +60,6 @@ __device__ __forceinline__ void run_session(const int (&pv)[kL], int L, uint32_t
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[k] += ta[k];
     }
-}
-
-// Buffer descriptors of the session stream: 32-bit offsets (no 64-bit
-// address arithmetic per load) and hardware bounds checks (a prefetch past
-// the last session reads zeros).
-struct Stream {
-    __amdgpu_buffer_rsrc_t pidx, mask, len;
-    uint32_t row_bytes;
-};
-
-__device__ __forceinline__ Stream make_stream(const KArgs& A) {
-    constexpr int kRawDwordFlags = 0x00020000;   // raw 32-bit buffer, no format conversion
-    Stream st;
-    st.pidx = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx, 0, (int)(uint32_t)(A.N * 40), kRawDwordFlags);
-    st.mask = __builtin_amdgcn_make_buffer_rsrc((void*)A.mask, 0, (int)(uint32_t)(A.N * 2), kRawDwordFlags);
-    st.len = A.len ? __builtin_amdgcn_make_buffer_rsrc((void*)A.len, 0, (int)(uint32_t)A.N, kRawDwordFlags) : st.mask;
-    st.row_bytes = (uint32_t)(A.N * 4);
-    return st;
-}
-
-// Loads one session's click bits, length and query-document indices
-// (rank-major layout: rank r of session s at pidx[r * N + s]).
-// Lanes past the end of their item (s >= s_end) re-read the item's last
-// session, whose cache lines are already fetched: no branch, no extra DRAM
-// traffic, and the values are never used.  s_end <= 0 (no item) reads past
-// the buffer, which returns zeros.
-template <bool VARLEN>
-__device__ __forceinline__ void load_session(const Stream& st, int s, int s_end, uint32_t& m, int& L, int (&pv)[kL]) {
-    const uint32_t o = (uint32_t)min(s, s_end - 1);
-    m = __builtin_amdgcn_raw_buffer_load_b16(st.mask, o * 2u, 0, 0);
-    L = VARLEN ? (int)__builtin_amdgcn_raw_buffer_load_b8(st.len, o, 0, 0) : kL;
-#pragma unroll
-    for (int r = 0; r < kL; r++) pv[r] = (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, o * 4u, r * st.row_bytes, 0);
 }
 
 // One lane walks the sessions s0 + tid, s0 + tid + blockDim, ...; the
@@ -173,18 +117,6 @@ __device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st
 #pragma unroll
         for (int r = 0; r < kL; r++) pv[r] = pv2[r];
     }
-}
-
-// Wave-level sum of a 64-bit integer (wave64).
-__device__ __forceinline__ long long wave_sum(long long v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Fixed point (32.32) plus a float base, rounded to float.
-__device__ __forceinline__ float fix_to_float(float base, unsigned long long v) {
-    return (float)((double)base + (double)(long long)v * kFixInv);
 }
 
 // Session paths, one kernel instantiation each (each gets its own register
@@ -647,61 +579,7 @@ __global__ void fill_kernel(float* p, float v, int64_t n) {
 // ===========================================================================
 using namespace mc;
 
-struct mc_model {
-    int type = 0;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    // options
-    int reduce_mode = MC_REDUCE_SEGMENTED;
-    int item_sessions = 8192;
-    int item_window = 1280;
-    int sort_by_lcr = 1;
-    int block = 256;
-    int direct = 0;
-    int persistent = 1;
-    int persist_grid = 0;     // resident workgroups of the persistent window kernel
-    // shape
-    int T = 1, K = 10, KACC = 10;
-    int64_t N = 0, n_qd = 0;
-    bool varlen = false;
-    bool initialized = false;
-    // device buffers
-    int32_t* d_pidx = nullptr;
-    uint16_t* d_mask = nullptr;
-    uint8_t* d_len = nullptr;
-    Item* d_items = nullptr;
-    int n_items = 0;
-    int n_win_items = 0;      // items [0, n_win_items) have an LDS window,
-    int n_glob_items = 0;     // then items merged with global atomics,
-    int n_dir_items = 0;      // then direct items (single-session groups)
-    int max_width = 0;
-    int64_t max_item_sessions = 0;
-    int32_t* d_shared_idx = nullptr;
-    int64_t n_shared = 0;
-    float* d_val[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [buf][t]
-    float* d_num[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    int2* d_counts = nullptr;
-    std::vector<int2> h_counts;   // host copy (occurrences, clicked) per entry
-    unsigned long long* d_gacc[2] = {nullptr, nullptr};
-    float* d_den_user[2] = {nullptr, nullptr};
-    int den_state[3] = {0, 0, 0};  // per table: 0 prior (2), 1 user-set, 2 M-step constant
-    float* d_sh_val[2] = {nullptr, nullptr};
-    float* d_sh_num[2] = {nullptr, nullptr};
-    float* d_sh_den[2] = {nullptr, nullptr};
-    double* d_sh_base = nullptr;
-    int32_t* d_sh_acc_map = nullptr;
-    unsigned long long* d_sh_acc = nullptr;
-    int cur = 0;
-    bool estep_pending = false;
-    size_t lds_bytes = 0;
-    size_t mem_bytes = 0;
-    std::vector<void*> allocations;
-    // graph cache, one slot per starting buffer parity
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_iters[2] = {0, 0};
-};
-
-namespace {
+namespace mc {
 
 const char* table_name(int type, int t) {
     switch (type) {
@@ -713,38 +591,9 @@ const char* table_name(int type, int t) {
     return "?";
 }
 
-void shape_of(int type, int* T, int* K, int* KACC) {
-    switch (type) {
-        case MC_PBM: *T = 1; *K = 10; *KACC = 10; break;
-        case MC_UBM: *T = 1; *K = 55; *KACC = 55; break;
-        case MC_DBN: *T = 2; *K = 1; *KACC = 2; break;
-        default: *T = 1; *K = 3; *KACC = 6; break;
-    }
-}
-
-template <class P>
-mc_status dalloc(mc_model* m, P** ptr, size_t bytes) {
-    void* p = nullptr;
-    if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
-        return e == hipErrorOutOfMemory ? MC_ERR_OOM : MC_ERR_HIP;
-    }
-    m->allocations.push_back(p);
-    m->mem_bytes += bytes;
-    *ptr = (P*)p;
-    return MC_OK;
-}
-
-#define MC_TRY(expr)                        \
-    do {                                    \
-        mc_status s_ = (expr);              \
-        if (s_ != MC_OK) return s_;         \
-    } while (0)
-
 void release(mc_model* m) {
+    if (m->adam_graph) { (void)hipGraphExecDestroy(m->adam_graph); m->adam_graph = nullptr; }
+    m->grad_ready = false;
     for (int b = 0; b < 2; b++)
         if (m->graph[b]) { (void)hipGraphExecDestroy(m->graph[b]); m->graph[b] = nullptr; }
     for (void* p : m->allocations) (void)hipFree(p);
@@ -758,6 +607,19 @@ mc_status fill(mc_model* m, float* p, float v, int64_t n) {
     hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, p, v, n);
     MC_HIP(hipGetLastError());
     return MC_OK;
+}
+
+}  // namespace mc
+
+namespace {
+
+void shape_of(int type, int* T, int* K, int* KACC) {
+    switch (type) {
+        case MC_PBM: *T = 1; *K = 10; *KACC = 10; break;
+        case MC_UBM: *T = 1; *K = 55; *KACC = 55; break;
+        case MC_DBN: *T = 2; *K = 1; *KACC = 2; break;
+        default: *T = 1; *K = 3; *KACC = 6; break;
+    }
 }
 
 KArgs make_args(mc_model* m, int cur, int write_num) {
@@ -1044,6 +906,10 @@ mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
         case MC_OPT_PERSISTENT:
             m->persistent = value != 0;
             break;
+        case MC_OPT_BATCHES:
+            MC_REQUIRE(value >= 1 && value <= 4096, "batches out of range [1, 4096]");
+            m->n_batches = (int)value;
+            break;
         case MC_OPT_BLOCK:
             MC_REQUIRE(value == 64 || value == 128 || value == 256 || value == 512, "block must be 64/128/256/512");
             m->block = (int)value;
@@ -1175,9 +1041,14 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
         int64_t count = 0;
         int32_t base = 0, width = 0, flags = 0;
         int kind = kKindWindow;
+        int64_t gstart = 0;     // first session (query order) of the item's first group
     };
     std::vector<Spec> specs;
-    std::vector<int32_t> shared_idx;
+    std::vector<int32_t> shared_idx, shared_batch;
+    // Minibatches of the gradient path: contiguous runs of whole query
+    // groups, cut by session count.
+    const int B = m->n_batches;
+    auto batch_of = [&](int64_t first_session) { return (int)std::min<int64_t>(B - 1, first_session * B / n); };
     bool structured = m->reduce_mode == MC_REDUCE_SEGMENTED;
     if (structured) {
         // Query groups: runs of equal query ids.  The segmented reduction needs
@@ -1223,12 +1094,14 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
                 const int64_t gs = g.s1 - g.s0;
                 const int64_t gw = g.hi - g.lo;
                 if (m->direct && gs == 1 && distinct(g.s0)) {
+                    if (dir.count == 0) dir.gstart = g.s0;
                     dir.ranges.push_back({g.s0, g.s1});
                     if (++dir.count == S_dir) close_dir();
                     continue;
                 }
                 if (gs <= S && gw <= W) {
-                    if (open && cur.count + gs <= S && (g.hi - cur.base) <= W) {
+                    if (open && cur.count + gs <= S && (g.hi - cur.base) <= W &&
+                        batch_of(g.s0) == batch_of(cur.gstart)) {
                         cur.ranges.push_back({g.s0, g.s1});
                         cur.count += gs;
                         cur.width = g.hi - cur.base;
@@ -1240,6 +1113,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
                         cur.base = g.lo;
                         cur.width = (int32_t)gw;
                         cur.flags = kItemInterior;
+                        cur.gstart = g.s0;
                         open = true;
                     }
                 } else {
@@ -1253,9 +1127,13 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
                         sl.base = g.lo;
                         sl.width = gw <= W ? (int32_t)gw : 0;
                         sl.kind = gw <= W ? kKindWindow : kKindGlobal;
+                        sl.gstart = g.s0;
                         specs.push_back(sl);
                     }
-                    for (int32_t p = g.lo; p < g.hi; p++) shared_idx.push_back(p);
+                    for (int32_t p = g.lo; p < g.hi; p++) {
+                        shared_idx.push_back(p);
+                        shared_batch.push_back(batch_of(g.s0));
+                    }
                 }
             }
             close();
@@ -1272,6 +1150,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
             c.ranges.push_back({a, std::min<int64_t>(n, a + S)});
             c.count = c.ranges[0].second - a;
             c.kind = kKindGlobal;
+            c.gstart = a;
             specs.push_back(c);
         }
     }
@@ -1295,7 +1174,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
             for (int64_t s = rg.first; s < rg.second; s++) order.push_back(s);
         if (m->sort_by_lcr)
             std::stable_sort(order.begin() + j0, order.end(), [&](int64_t a, int64_t b) { return lcr[a] < lcr[b]; });
-        items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, 0});
+        items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, batch_of(sp.gstart)});
         if (sp.kind == kKindWindow && m->reduce_mode == MC_REDUCE_SEGMENTED) m->n_win_items++;
         else if (sp.kind == kKindDirect) m->n_dir_items++;
         else m->n_glob_items++;
@@ -1324,6 +1203,42 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     }
     m->n_items = (int)items.size();
     m->n_shared = (int64_t)shared_idx.size();
+
+    // Minibatch lists of the gradient path: per batch its window items and
+    // its global items (item indices), then the same for every item; split
+    // entries ordered by batch.
+    std::vector<int32_t> lists;
+    {
+        std::vector<size_t> perm(shared_idx.size());
+        std::iota(perm.begin(), perm.end(), (size_t)0);
+        std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return shared_batch[a] < shared_batch[b]; });
+        std::vector<int32_t> sorted(shared_idx.size());
+        for (size_t i = 0; i < perm.size(); i++) sorted[i] = shared_idx[perm[i]];
+        m->shared_off.assign((size_t)B + 1, 0);
+        m->shared_cnt.assign((size_t)B + 1, 0);
+        for (size_t i = 0; i < perm.size(); i++) m->shared_cnt[(size_t)shared_batch[perm[i]]]++;
+        for (int b = 1; b < B; b++) m->shared_off[(size_t)b] = m->shared_off[(size_t)b - 1] + m->shared_cnt[(size_t)b - 1];
+        m->shared_off[(size_t)B] = 0;
+        m->shared_cnt[(size_t)B] = (int64_t)sorted.size();
+        shared_idx.swap(sorted);
+        m->win_off.assign((size_t)B + 1, 0);
+        m->win_cnt.assign((size_t)B + 1, 0);
+        m->glob_off.assign((size_t)B + 1, 0);
+        m->glob_cnt.assign((size_t)B + 1, 0);
+        m->batch_sessions.assign((size_t)B, 0);
+        for (const Item& it : items) m->batch_sessions[(size_t)it.pad] += it.s1 - it.s0;
+        const int g0 = m->n_win_items, g1 = m->n_win_items + m->n_glob_items;
+        for (int b = 0; b <= B; b++) {
+            m->win_off[(size_t)b] = (int64_t)lists.size();
+            for (int i = 0; i < g0; i++)
+                if (b == B || items[(size_t)i].pad == b) lists.push_back(i);
+            m->win_cnt[(size_t)b] = (int64_t)lists.size() - m->win_off[(size_t)b];
+            m->glob_off[(size_t)b] = (int64_t)lists.size();
+            for (int i = g0; i < g1; i++)
+                if (b == B || items[(size_t)i].pad == b) lists.push_back(i);
+            m->glob_cnt[(size_t)b] = (int64_t)lists.size() - m->glob_off[(size_t)b];
+        }
+    }
     m->max_item_sessions = 0;
     for (const Item& it : items) m->max_item_sessions = std::max<int64_t>(m->max_item_sessions, it.s1 - it.s0);
     int max_width = 0;
@@ -1375,6 +1290,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     if (m->varlen) MC_TRY(dalloc(m, &m->d_len, h_len.size()));
     MC_TRY(dalloc(m, &m->d_items, items.size() * sizeof(Item)));
     MC_TRY(dalloc(m, &m->d_shared_idx, shared_idx.size() * 4));
+    MC_TRY(dalloc(m, &m->d_lists, lists.size() * 4));
     for (int t = 0; t < T; t++) {
         for (int b = 0; b < 2; b++) {
             MC_TRY(dalloc(m, &m->d_val[b][t], (size_t)n_qd * 4));
@@ -1400,6 +1316,8 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     if (!shared_idx.empty())
         MC_HIP(hipMemcpyAsync(m->d_shared_idx, shared_idx.data(), shared_idx.size() * 4, hipMemcpyHostToDevice,
                               m->stream));
+    if (!lists.empty())
+        MC_HIP(hipMemcpyAsync(m->d_lists, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_counts, counts.data(), counts.size() * 8, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_sh_base, sh_base.data(), sh_base.size() * 8, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_sh_acc_map, sh_map.data(), sh_map.size() * 4, hipMemcpyHostToDevice, m->stream));
@@ -1570,6 +1488,12 @@ mc_status mc_get_parameters(mc_model* m, int t, float* num, float* den) {
     MC_HIP(hipStreamSynchronize(m->stream));
     const bool shared = t == m->T;
     const int64_t n = shared ? m->K : m->n_qd;
+    if (!shared && m->den_state[t] == 3) {
+        // Values set by the gradient path: (value, 1).
+        if (num) MC_HIP(hipMemcpy(num, m->d_val[m->cur][t], n * 4, hipMemcpyDeviceToHost));
+        if (den) std::fill(den, den + n, 1.f);
+        return MC_OK;
+    }
     if (num) MC_HIP(hipMemcpy(num, shared ? m->d_sh_num[m->cur] : m->d_num[m->cur][t], n * 4, hipMemcpyDeviceToHost));
     if (den) {
         if (shared) {

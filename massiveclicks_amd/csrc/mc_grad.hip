@@ -1,0 +1,729 @@
+// mc_grad.hip -- log-likelihood gradient kernels and the Adam fit.
+//
+// The objective is the click log-likelihood the reference evaluates
+// (get_log_conditional_click_probs, e.g. dbn.cu:611-639, summed over the
+// ranks of every training session), as a function of logit-parameterised
+// tables: value = sigmoid(logit) for attractiveness, examination,
+// satisfaction and continuation.  One lane runs one session: the forward
+// chain is recomputed in registers and differentiated in reverse, so no
+// per-session state is stored.  Gradients are summed per query-document
+// entry with the same work items, LDS windows and order-independent 64-bit
+// fixed-point sums as the EM E-step (two's complement: contributions may be
+// negative), so they are deterministic.
+//
+// The Adam fit maximises log-likelihood + log-prior, where the prior of each
+// parameter is Beta(1 + w, 1 + w) with w the pseudo-count weight that makes
+// plain EM's update a MAP-EM step: w = 1 for every parameter, plus the
+// never-written temporaries the reference counts as (1,2) (DBN satisfaction:
+// unclicked occurrences; PBM/UBM examination: sessions not writing the slot).
+// Minibatches are runs of whole query groups, so an entry's gradient is
+// complete whenever its batch runs; the shared (rank-indexed) tables get the
+// batch gradient scaled to the whole training set.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mclicks.h"
+#include "mc_internal.h"
+#include "mc_kernels.h"
+#include "mc_model.h"
+
+namespace mc {
+namespace {
+
+constexpr float kLogitMax = 13.8f;   // sigmoid(13.8) = 1 - 1.0e-6, the Param::value clamp
+
+struct GArgs {
+    const int32_t* pidx;
+    const uint16_t* mask;
+    const uint8_t* len;
+    int64_t N;
+    const Item* items;
+    const int32_t* lists;
+    float* val[2];                 // per-qd values (read; written by Adam)
+    float* logit[2];
+    float* am[2];
+    float* av[2];
+    float* grad[2];                // gradient output (GRAD mode)
+    unsigned long long* gacc[2];   // fixed-point sums of split entries
+    const int2* counts;
+    float* sh_val;
+    float* sh_num;
+    float* sh_den;
+    float* sh_logit;
+    float* sh_am;
+    float* sh_av;
+    float* sh_grad;
+    const double* sh_prior;
+    unsigned long long* sh_acc;
+    double* ll_part;               // per item
+    const int32_t* shared_idx;
+    int32_t* steps;                // Adam steps per batch, then of the shared tables
+    int32_t K, kacc, max_width, batch, n_batches, use_prior;
+    float lr, b1, b2, eps;
+    double sh_scale;               // training sessions / batch sessions
+};
+
+// Signed 32.32 fixed point (two's complement), saturating at +-2^30.
+__device__ __forceinline__ long long to_fix_signed(float x) {
+    x = fminf(fmaxf(x, -1073741824.f), 1073741824.f);
+    const float fl = floorf(x);
+    const int hi = (int)fl;
+    const uint32_t lo = (uint32_t)((x - fl) * kFixScale);
+    return (long long)(((unsigned long long)(long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double fix_to_double(unsigned long long v) { return (double)(long long)v * kFixInv; }
+
+template <int T>
+struct EmitLdsSigned {
+    unsigned long long* acc;
+    __device__ __forceinline__ void add(int t, int, int p, float x) {
+        atomicAdd(&acc[p * T + t], (unsigned long long)to_fix_signed(x));
+    }
+};
+struct EmitGlobalSigned {
+    unsigned long long* g0;
+    unsigned long long* g1;
+    __device__ __forceinline__ void add(int t, int, int p, float x) {
+        atomicAdd(t == 0 ? &g0[p] : &g1[p], (unsigned long long)to_fix_signed(x));
+    }
+};
+
+__device__ __forceinline__ float sigmoid(float x) { return 1.f / (1.f + __expf(-x)); }
+
+// ---------------------------------------------------------------------------
+// Per-session gradients (logit space).  Clicked occurrences of an
+// attractiveness entry contribute exactly 1 - a each (d log a / d logit);
+// that term is added per entry from its click count when the entry is
+// finalised, so only the other contributions are emitted.  acc[k] collects
+// the gradient of shared-table entry k, ll the session's log-likelihood.
+// ---------------------------------------------------------------------------
+
+// PBM/UBM: log P(C) = sum_r c log(a e) + (1 - c) log(1 - a e), e from
+// rank r (PBM) or slot r(r+1)/2 + prev (UBM, ubm.cu:201).
+template <int MODEL, bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void grad_pbm_ubm(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
+                                             const float* sh, float (&acc)[55], float& ll) {
+    int slot[kL];
+    if (MODEL == kUBM) ubm_slots(mm, slot);
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        if (VARLEN && r >= L) continue;
+        const int p = pv[r];
+        const float a = src.get(0, p);
+        const float e = MODEL == kUBM ? sh[slot[r]] : sh[r];
+        const float ae = a * e;
+        float ge;
+        if ((mm >> r) & 1u) {
+            ll += __logf(ae);
+            ge = 1.f - e;
+        } else {
+            const float q = 1.f - ae;
+            ll += __logf(q);
+            const float w = ae * __builtin_amdgcn_rcpf(q);
+            em.add(0, r, p, -w * (1.f - a));
+            ge = -w * (1.f - e);
+        }
+        if (MODEL == kUBM) {
+            const int row = r * (r + 1) / 2;
+#pragma unroll
+            for (int j = 0; j < r; j++) acc[row + j] += (slot[r] == row + j) ? ge : 0.f;
+            acc[row + kL - 1] += (slot[r] == row + kL - 1) ? ge : 0.f;
+        } else {
+            acc[r] += ge;
+        }
+    }
+}
+
+// DBN / CCM: the conditional chain of dbn.cu:611-639 / ccm.cu:604-630.
+// Forward: ex_0 = 1; click: p = a ex, ex' = g(1-s) [CCM: t2(1-a) + t3 a];
+// no click: p = 1 - a ex, ex' = ex g(1-a)/p [CCM: t1].  Reverse sweep with
+// dex = d LL / d ex_{r+1}:
+//   click     d/da = 1/a (+ dex (t3 - t2) for the CCM), d/ds = -g dex,
+//             d/dg += dex (1-s), d LL/d ex_r = 1/ex_r;
+//   no click  q = 1 - a ex: d/da = -ex/q + dex ex g (ex-1)/q^2,
+//             d/dg += dex ex (1-a)/q, d LL/d ex_r = -a/q + dex g (1-a)/q^2.
+template <int MODEL, bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void grad_cascade(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
+                                             const float* sh, float (&acc)[55], float& ll) {
+#pragma clang fp contract(fast)
+    const float g = sh[0];                                 // DBN gamma / CCM tau1
+    const float t2 = MODEL == kCCM ? sh[1] : 0.f, t3 = MODEL == kCCM ? sh[2] : 0.f;
+    float exv[kL];
+    float e = 1.f;
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        exv[r] = e;
+        if (VARLEN && r >= L) continue;
+        const float a = src.get(0, pv[r]);
+        if ((mm >> r) & 1u) {
+            ll += __logf(a * e);
+            e = MODEL == kDBN ? g * (1.f - src.get(1, pv[r])) : t2 * (1.f - a) + t3 * a;
+        } else {
+            const float q = 1.f - a * e;
+            ll += __logf(q);
+            e = e * g * (1.f - a) * __builtin_amdgcn_rcpf(q);
+        }
+    }
+    __asm__ volatile("" ::: "memory");
+    float dex = 0.f, dg = 0.f, d2 = 0.f, d3 = 0.f;
+#pragma unroll
+    for (int r = kL - 1; r >= 0; r--) {
+        if (VARLEN && r >= L) continue;
+        const float E = exv[r];
+        const int p = pv[r];
+        const float a = src.get(0, p);
+        if ((mm >> r) & 1u) {
+            if (MODEL == kDBN) {
+                const float s = src.get(1, p);
+                em.add(1, r, p, -dex * g * s * (1.f - s));
+                dg += dex * (1.f - s);
+            } else {
+                em.add(0, r, p, dex * (t3 - t2) * a * (1.f - a));
+                d2 += dex * (1.f - a);
+                d3 += dex * a;
+            }
+            dex = __builtin_amdgcn_rcpf(E);
+        } else {
+            const float iq = __builtin_amdgcn_rcpf(1.f - a * E);
+            const float iq2 = iq * iq;
+            const float da = -E * iq + dex * E * g * (E - 1.f) * iq2;
+            dg += dex * E * (1.f - a) * iq;
+            dex = -a * iq + dex * g * (1.f - a) * iq2;
+            em.add(0, r, p, da * a * (1.f - a));
+        }
+    }
+    acc[0] += dg * g * (1.f - g);
+    if (MODEL == kCCM) {
+        acc[1] += d2 * t2 * (1.f - t2);
+        acc[2] += d3 * t3 * (1.f - t3);
+    }
+}
+
+template <int MODEL, bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void grad_session(const int (&pv)[kL], int L, uint32_t m, const Src& src, Emit& em,
+                                             const float* sh, float (&acc)[55], float& ll) {
+    const uint32_t mm = m & ((1u << L) - 1u);
+    if (MODEL == kPBM || MODEL == kUBM) grad_pbm_ubm<MODEL, VARLEN>(pv, L, mm, src, em, sh, acc, ll);
+    else grad_cascade<MODEL, VARLEN>(pv, L, mm, src, em, sh, acc, ll);
+}
+
+// Adam (maximising): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
+// logit += lr/(1-b1^t) m / (sqrt(v/(1-b2^t)) + eps), clamped so that the
+// value stays inside Param::value's range.
+struct AdamStep {
+    float lr_bc1, inv_sqrt_bc2, b1, b2, eps;
+    __device__ __forceinline__ float apply(float& logit, float& m, float& v, float g) const {
+        m = b1 * m + (1.f - b1) * g;
+        v = b2 * v + (1.f - b2) * g * g;
+        const float d = sqrtf(v) * inv_sqrt_bc2 + eps;
+        logit = fminf(fmaxf(logit + lr_bc1 * m / d, -kLogitMax), kLogitMax);
+        return sigmoid(logit);
+    }
+};
+
+__device__ __forceinline__ AdamStep adam_step(const GArgs& G, int t_step) {
+    AdamStep s;
+    const float bc1 = 1.f - powf(G.b1, (float)t_step);
+    const float bc2 = 1.f - powf(G.b2, (float)t_step);
+    s.lr_bc1 = G.lr / bc1;
+    s.inv_sqrt_bc2 = 1.f / sqrtf(bc2);
+    s.b1 = G.b1;
+    s.b2 = G.b2;
+    s.eps = G.eps;
+    return s;
+}
+
+// Gradient of per-qd entry p of table t from its fixed-point sum: adds the
+// clicked occurrences' 1 - a (attractiveness) and, for Adam, the prior.
+template <int MODEL>
+__device__ __forceinline__ float entry_gradient(int t, double sum, int2 oc, float val, bool prior) {
+    double g = sum;
+    if (t == 0) g += (double)oc.y * (1.0 - (double)val);
+    if (prior) {
+        const double w = 1.0 + ((MODEL == kDBN && t == 1) ? (double)(oc.x - oc.y) : 0.0);
+        g += w * (1.0 - 2.0 * (double)val);
+    }
+    return (float)g;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One workgroup per listed item: the sessions' gradients, reduced in the
+// item's LDS window (WIN) or with global fixed-point atomics, then
+// finalised for the entries the item owns: written out (ADAM = false) or
+// applied by Adam in place.
+template <int MODEL, bool VARLEN, bool WIN, bool ADAM>
+__global__ void __launch_bounds__(256) grad_item_kernel(GArgs G, int64_t list_off) {
+    constexpr int T = Shape<MODEL>::T;
+    constexpr int K = Shape<MODEL>::K;
+    extern __shared__ unsigned long long lds[];
+    __shared__ float sh[K < 64 ? 64 : K];
+    __shared__ long long red[K][4];
+    __shared__ double lred[4];
+
+    const int item_idx = G.lists[list_off + blockIdx.x];
+    const Item it = G.items[item_idx];
+    const int tid = threadIdx.x;
+    constexpr int NT = 256;
+    const int W = WIN ? it.width : 0;
+    unsigned long long* lacc = lds;
+    float* lval = (float*)(lds + T * G.max_width);
+
+    KArgs ka;
+    ka.pidx = G.pidx;
+    ka.mask = G.mask;
+    ka.len = G.len;
+    ka.N = G.N;
+    const Stream st = make_stream(ka);
+    uint32_t m0;
+    int L0;
+    int pv0[kL];
+    load_session<VARLEN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    for (int k = tid; k < K; k += NT) sh[k] = G.sh_val[k];
+    if (WIN) {
+        for (int i = tid; i < W; i += NT) {
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                lacc[i * T + t] = 0ull;
+                lval[i * T + t] = G.val[t][it.base + i];
+            }
+        }
+    }
+    __syncthreads();
+
+    float acc[55];
+#pragma unroll
+    for (int k = 0; k < 55; k++) acc[k] = 0.f;
+    float ll = 0.f;
+    {
+        const SrcLds<T> lsrc{lval};
+        const SrcGlobal gsrc{G.val[0], G.val[1]};
+        EmitLdsSigned<T> el{lacc};
+        EmitGlobalSigned eg{G.gacc[0], G.gacc[1]};
+        for (int s = it.s0 + tid; s < it.s1; s += NT) {
+            uint32_t m2;
+            int L2;
+            int pv2[kL];
+            load_session<VARLEN>(st, s + NT, it.s1, m2, L2, pv2);
+            if (WIN) grad_session<MODEL, VARLEN>(pv0, L0, m0, lsrc, el, sh, acc, ll);
+            else grad_session<MODEL, VARLEN>(pv0, L0, m0, gsrc, eg, sh, acc, ll);
+            m0 = m2;
+            L0 = L2;
+#pragma unroll
+            for (int r = 0; r < kL; r++) pv0[r] = pv2[r];
+        }
+    }
+
+    // Shared tables and the log-likelihood: wave sums, then one atomic per
+    // accumulator and item.
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const long long v = wave_sum(to_fix_signed(acc[k]));
+        if (lane == 0) red[k][wave] = v;
+    }
+    const double lw = wave_sum_f64((double)ll);
+    if (lane == 0) lred[wave] = lw;
+    __syncthreads();
+    for (int k = tid; k < K; k += NT) {
+        const long long v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        if (v != 0) atomicAdd(&G.sh_acc[k], (unsigned long long)v);
+    }
+    if (!ADAM && tid == 0) G.ll_part[item_idx] = ((lred[0] + lred[1]) + lred[2]) + lred[3];
+
+    if (WIN) {
+        const bool interior = (it.flags & kItemInterior) != 0;
+        AdamStep ad;
+        if (ADAM && interior) ad = adam_step(G, G.steps[G.batch] + 1);
+        for (int i = tid; i < W; i += NT) {
+            const int p = it.base + i;
+            const int2 oc = interior ? G.counts[p] : make_int2(0, 0);
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                const unsigned long long v = lacc[i * T + t];
+                if (interior) {
+                    const float g = entry_gradient<MODEL>(t, fix_to_double(v), oc, lval[i * T + t], ADAM && G.use_prior);
+                    if (ADAM) {
+                        G.val[t][p] = ad.apply(G.logit[t][p], G.am[t][p], G.av[t][p], g);
+                    } else {
+                        G.grad[t][p] = g;
+                    }
+                } else if (v != 0ull) {
+                    atomicAdd(&G.gacc[t][p], v);
+                }
+            }
+        }
+    }
+}
+
+// Block 0: the shared tables (gradient scaled to the training set, prior,
+// Adam, step counters); other blocks: the listed split entries.
+template <int MODEL, bool ADAM>
+__global__ void __launch_bounds__(256) grad_merge_kernel(GArgs G, int64_t shared_off, int64_t shared_cnt) {
+    constexpr int T = Shape<MODEL>::T;
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0) {
+        if (tid < G.K) {
+            const double sum = fix_to_double(G.sh_acc[tid]);
+            if (ADAM) {
+                const float val = G.sh_val[tid];
+                double g = sum * G.sh_scale;
+                if (G.use_prior) g += G.sh_prior[tid] * (1.0 - 2.0 * (double)val);
+                const AdamStep ad = adam_step(G, G.steps[G.n_batches] + 1);
+                const float nv = ad.apply(G.sh_logit[tid], G.sh_am[tid], G.sh_av[tid], (float)g);
+                G.sh_val[tid] = nv;
+                G.sh_num[tid] = nv;
+                G.sh_den[tid] = 1.f;
+            } else {
+                G.sh_grad[tid] = (float)sum;
+            }
+        }
+        __syncthreads();
+        for (int j = tid; j < G.kacc; j += blockDim.x) G.sh_acc[j] = 0ull;
+        return;
+    }
+    const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
+    if (i >= shared_cnt) return;
+    const int p = G.shared_idx[shared_off + i];
+    const int2 oc = G.counts[p];
+    AdamStep ad;
+    if (ADAM) ad = adam_step(G, G.steps[G.batch] + 1);
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const float val = G.val[t][p];
+        const float g = entry_gradient<MODEL>(t, fix_to_double(G.gacc[t][p]), oc, val, ADAM && G.use_prior);
+        if (ADAM) G.val[t][p] = ad.apply(G.logit[t][p], G.am[t][p], G.av[t][p], g);
+        else G.grad[t][p] = g;
+        G.gacc[t][p] = 0ull;
+    }
+}
+
+// logit = log(v / (1 - v)) of every entry; moments to zero.
+__global__ void logit_init_kernel(const float* val, float* logit, float* am, float* av, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = val[i];
+    logit[i] = fminf(fmaxf(logf(v) - logf(1.f - v), -kLogitMax), kLogitMax);
+    am[i] = 0.f;
+    av[i] = 0.f;
+}
+
+__global__ void sigmoid_kernel(const float* logit, float* val, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) val[i] = sigmoid(logit[i]);
+}
+
+// Counts one Adam step of `batch` and of the shared tables; runs after the
+// step's kernels, which read the counts.
+__global__ void advance_steps_kernel(int32_t* steps, int batch, int n_batches) {
+    if (threadIdx.x == 0) {
+        steps[batch] += 1;
+        steps[n_batches] += 1;
+    }
+}
+
+__global__ void zero_steps_kernel(int32_t* steps, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) steps[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+mc_status check_grad_support(mc_model* m) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    if (m->reduce_mode != MC_REDUCE_SEGMENTED || m->n_dir_items > 0) {
+        set_error("the gradient path needs the segmented layout without direct items");
+        return MC_ERR_UNSUPPORTED;
+    }
+    MC_REQUIRE(!m->estep_pending, "an E-step is pending");
+    return MC_OK;
+}
+
+mc_status ensure_grad_buffers(mc_model* m) {
+    if (m->grad_ready) return MC_OK;
+    const int T = m->T, K = m->K;
+    for (int t = 0; t < T; t++) {
+        MC_TRY(dalloc(m, &m->d_logit[t], (size_t)m->n_qd * 4));
+        MC_TRY(dalloc(m, &m->d_am[t], (size_t)m->n_qd * 4));
+        MC_TRY(dalloc(m, &m->d_av[t], (size_t)m->n_qd * 4));
+        MC_TRY(dalloc(m, &m->d_grad[t], (size_t)m->n_qd * 4));
+    }
+    MC_TRY(dalloc(m, &m->d_sh_logit, (size_t)K * 4));
+    MC_TRY(dalloc(m, &m->d_sh_am, (size_t)K * 4));
+    MC_TRY(dalloc(m, &m->d_sh_av, (size_t)K * 4));
+    MC_TRY(dalloc(m, &m->d_sh_grad, (size_t)K * 4));
+    MC_TRY(dalloc(m, &m->d_sh_prior, (size_t)K * 8));
+    MC_TRY(dalloc(m, &m->d_ll_part, (size_t)m->n_items * 8));
+    MC_TRY(dalloc(m, &m->d_steps, (size_t)(m->n_batches + 1) * 4));
+    // Prior weights of the shared tables: 1 + the sessions that leave the
+    // slot's temporary at (1,2) (the phantom count in sh_base).
+    std::vector<double> base(2 * (size_t)K), prior((size_t)K);
+    MC_HIP(hipMemcpy(base.data(), m->d_sh_base, base.size() * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; k++) {
+        // PBM/UBM: the constant denominator of slot k is 2 + w + 2 ph with
+        // w writers and ph = N - w sessions leaving it at (1,2), so
+        // ph = den - 2 - N.
+        double ph = 0.0;
+        if (m->type == MC_PBM || m->type == MC_UBM) ph = base[2 * (size_t)k + 1] - 2.0 - (double)m->N;
+        prior[(size_t)k] = 1.0 + ph;
+    }
+    MC_HIP(hipMemcpy(m->d_sh_prior, prior.data(), prior.size() * 8, hipMemcpyHostToDevice));
+    m->grad_ready = true;
+    return MC_OK;
+}
+
+GArgs make_gargs(mc_model* m, int batch) {
+    GArgs G;
+    std::memset(&G, 0, sizeof G);
+    G.pidx = m->d_pidx;
+    G.mask = m->d_mask;
+    G.len = m->varlen ? m->d_len : nullptr;
+    G.N = m->N;
+    G.items = m->d_items;
+    G.lists = m->d_lists;
+    for (int t = 0; t < 2; t++) {
+        const int tt = t < m->T ? t : 0;
+        G.val[t] = m->d_val[m->cur][tt];
+        G.logit[t] = m->d_logit[tt];
+        G.am[t] = m->d_am[tt];
+        G.av[t] = m->d_av[tt];
+        G.grad[t] = m->d_grad[tt];
+        G.gacc[t] = m->d_gacc[tt];
+    }
+    G.counts = m->d_counts;
+    G.sh_val = m->d_sh_val[m->cur];
+    G.sh_num = m->d_sh_num[m->cur];
+    G.sh_den = m->d_sh_den[m->cur];
+    G.sh_logit = m->d_sh_logit;
+    G.sh_am = m->d_sh_am;
+    G.sh_av = m->d_sh_av;
+    G.sh_grad = m->d_sh_grad;
+    G.sh_prior = m->d_sh_prior;
+    G.sh_acc = m->d_sh_acc;
+    G.ll_part = m->d_ll_part;
+    G.shared_idx = m->d_shared_idx;
+    G.steps = m->d_steps;
+    G.K = m->K;
+    G.kacc = m->KACC;
+    G.max_width = m->max_width;
+    G.batch = batch < 0 ? 0 : batch;
+    G.n_batches = m->n_batches;
+    G.use_prior = m->adam_prior;
+    G.lr = m->adam_lr;
+    G.b1 = m->adam_b1;
+    G.b2 = m->adam_b2;
+    G.eps = m->adam_eps;
+    const int64_t nb = batch < 0 ? m->N : m->batch_sessions[(size_t)batch];
+    G.sh_scale = nb > 0 ? (double)m->N / (double)nb : 1.0;
+    return G;
+}
+
+template <int MODEL, bool ADAM>
+void launch_grad_t(mc_model* m, const GArgs& G, int batch) {
+    const size_t b = batch < 0 ? (size_t)m->n_batches : (size_t)batch;
+    const int64_t nw = m->win_cnt[b], ng = m->glob_cnt[b];
+    if (nw > 0) {
+        if (m->varlen)
+            hipLaunchKernelGGL((grad_item_kernel<MODEL, true, true, ADAM>), dim3((unsigned)nw), dim3(256), m->lds_bytes,
+                               m->stream, G, m->win_off[b]);
+        else
+            hipLaunchKernelGGL((grad_item_kernel<MODEL, false, true, ADAM>), dim3((unsigned)nw), dim3(256),
+                               m->lds_bytes, m->stream, G, m->win_off[b]);
+    }
+    if (ng > 0) {
+        if (m->varlen)
+            hipLaunchKernelGGL((grad_item_kernel<MODEL, true, false, ADAM>), dim3((unsigned)ng), dim3(256), 0, m->stream,
+                               G, m->glob_off[b]);
+        else
+            hipLaunchKernelGGL((grad_item_kernel<MODEL, false, false, ADAM>), dim3((unsigned)ng), dim3(256), 0,
+                               m->stream, G, m->glob_off[b]);
+    }
+    const int64_t so = m->shared_off[b], sc = m->shared_cnt[b];
+    const unsigned grid = (unsigned)(1 + (sc + 255) / 256);
+    hipLaunchKernelGGL((grad_merge_kernel<MODEL, ADAM>), dim3(grid), dim3(256), 0, m->stream, G, so, sc);
+    if (ADAM) hipLaunchKernelGGL(advance_steps_kernel, dim3(1), dim3(64), 0, m->stream, G.steps, G.batch, G.n_batches);
+}
+
+mc_status launch_grad(mc_model* m, int batch, bool adam) {
+    const GArgs G = make_gargs(m, batch);
+    switch (m->type) {
+        case MC_PBM: adam ? launch_grad_t<kPBM, true>(m, G, batch) : launch_grad_t<kPBM, false>(m, G, batch); break;
+        case MC_UBM: adam ? launch_grad_t<kUBM, true>(m, G, batch) : launch_grad_t<kUBM, false>(m, G, batch); break;
+        case MC_DBN: adam ? launch_grad_t<kDBN, true>(m, G, batch) : launch_grad_t<kDBN, false>(m, G, batch); break;
+        default: adam ? launch_grad_t<kCCM, true>(m, G, batch) : launch_grad_t<kCCM, false>(m, G, batch); break;
+    }
+    MC_HIP(hipGetLastError());
+    return MC_OK;
+}
+
+void mark_value_only(mc_model* m) {
+    for (int t = 0; t < m->T; t++) m->den_state[t] = 3;
+}
+
+}  // namespace
+}  // namespace mc
+
+using namespace mc;
+
+extern "C" {
+
+mc_status mc_grad_init(mc_model* m) {
+    MC_TRY(check_grad_support(m));
+    MC_HIP(hipSetDevice(m->device));
+    MC_TRY(ensure_grad_buffers(m));
+    for (int t = 0; t < m->T; t++)
+        hipLaunchKernelGGL(logit_init_kernel, dim3((unsigned)((m->n_qd + 255) / 256)), dim3(256), 0, m->stream,
+                           m->d_val[m->cur][t], m->d_logit[t], m->d_am[t], m->d_av[t], m->n_qd);
+    hipLaunchKernelGGL(logit_init_kernel, dim3(1), dim3(256), 0, m->stream, m->d_sh_val[m->cur], m->d_sh_logit,
+                       m->d_sh_am, m->d_sh_av, (int64_t)m->K);
+    hipLaunchKernelGGL(zero_steps_kernel, dim3(1 + m->n_batches / 256), dim3(256), 0, m->stream, m->d_steps,
+                       m->n_batches + 1);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipStreamSynchronize(m->stream));
+    m->adam_next_batch = 0;
+    return MC_OK;
+}
+
+mc_status mc_loglik_gradient(mc_model* m, int batch, double* loglik, float* const* grad) {
+    MC_TRY(check_grad_support(m));
+    MC_REQUIRE(batch >= -1 && batch < m->n_batches, "batch out of range");
+    MC_HIP(hipSetDevice(m->device));
+    MC_TRY(ensure_grad_buffers(m));
+    for (int t = 0; t < m->T; t++) MC_TRY(fill(m, m->d_grad[t], 0.f, m->n_qd));
+    MC_TRY(fill(m, (float*)m->d_ll_part, 0.f, 2 * (int64_t)m->n_items));
+    MC_TRY(launch_grad(m, batch, false));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    if (loglik) {
+        std::vector<double> part((size_t)m->n_items);
+        MC_HIP(hipMemcpy(part.data(), m->d_ll_part, part.size() * 8, hipMemcpyDeviceToHost));
+        double s = 0.0;
+        for (double v : part) s += v;
+        *loglik = s;
+    }
+    if (grad) {
+        for (int t = 0; t < m->T; t++)
+            if (grad[t]) MC_HIP(hipMemcpy(grad[t], m->d_grad[t], (size_t)m->n_qd * 4, hipMemcpyDeviceToHost));
+        if (grad[m->T]) MC_HIP(hipMemcpy(grad[m->T], m->d_sh_grad, (size_t)m->K * 4, hipMemcpyDeviceToHost));
+    }
+    return MC_OK;
+}
+
+mc_status mc_adam_configure(mc_model* m, float lr, float beta1, float beta2, float eps, int use_prior) {
+    MC_REQUIRE(m, "null model");
+    MC_REQUIRE(lr > 0.f && beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f,
+               "invalid Adam hyper-parameters");
+    m->adam_lr = lr;
+    m->adam_b1 = beta1;
+    m->adam_b2 = beta2;
+    m->adam_eps = eps;
+    m->adam_prior = use_prior != 0;
+    if (m->adam_graph) {
+        (void)hipGraphExecDestroy(m->adam_graph);
+        m->adam_graph = nullptr;
+    }
+    return MC_OK;
+}
+
+mc_status mc_adam_steps(mc_model* m, int n_steps, int use_graph) {
+    MC_TRY(check_grad_support(m));
+    MC_REQUIRE(m->grad_ready, "call mc_grad_init first");
+    MC_REQUIRE(n_steps >= 0, "n_steps must be >= 0");
+    MC_HIP(hipSetDevice(m->device));
+    const int B = m->n_batches;
+    int done = 0;
+    // Whole epochs from batch 0 replay one captured graph of B steps.
+    while (use_graph && m->adam_next_batch == 0 && n_steps - done >= B) {
+        if (!m->adam_graph || m->adam_graph_first != m->cur) {
+            if (m->adam_graph) {
+                (void)hipGraphExecDestroy(m->adam_graph);
+                m->adam_graph = nullptr;
+            }
+            hipGraph_t g;
+            MC_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+            mc_status st = MC_OK;
+            for (int b = 0; b < B && st == MC_OK; b++) st = launch_grad(m, b, true);
+            hipError_t e = hipStreamEndCapture(m->stream, &g);
+            if (st != MC_OK) return st;
+            MC_HIP(e);
+            e = hipGraphInstantiate(&m->adam_graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            MC_HIP(e);
+            m->adam_graph_first = m->cur;
+        }
+        MC_HIP(hipGraphLaunch(m->adam_graph, m->stream));
+        done += B;
+    }
+    for (; done < n_steps; done++) {
+        MC_TRY(launch_grad(m, m->adam_next_batch, true));
+        m->adam_next_batch = (m->adam_next_batch + 1) % B;
+    }
+    MC_HIP(hipStreamSynchronize(m->stream));
+    mark_value_only(m);
+    return MC_OK;
+}
+
+mc_status mc_adam_steps_timed(mc_model* m, int n_steps, double* ms) {
+    MC_REQUIRE(m && ms, "null argument");
+    MC_HIP(hipSetDevice(m->device));
+    hipEvent_t a, b;
+    MC_HIP(hipEventCreate(&a));
+    MC_HIP(hipEventCreate(&b));
+    MC_HIP(hipEventRecord(a, m->stream));
+    mc_status st = mc_adam_steps(m, n_steps, 1);
+    MC_HIP(hipEventRecord(b, m->stream));
+    MC_HIP(hipEventSynchronize(b));
+    float t = 0.f;
+    MC_HIP(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = t;
+    return st;
+}
+
+mc_status mc_get_logits(mc_model* m, int t, float* logits) {
+    MC_REQUIRE(m && m->initialized && logits && t >= 0 && t <= m->T, "invalid argument");
+    MC_REQUIRE(m->grad_ready, "call mc_grad_init first");
+    MC_HIP(hipSetDevice(m->device));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    const bool sh = t == m->T;
+    MC_HIP(hipMemcpy(logits, sh ? m->d_sh_logit : m->d_logit[t], (size_t)(sh ? m->K : m->n_qd) * 4,
+                     hipMemcpyDeviceToHost));
+    return MC_OK;
+}
+
+mc_status mc_set_logits(mc_model* m, int t, const float* logits) {
+    MC_REQUIRE(m && m->initialized && logits && t >= 0 && t <= m->T, "invalid argument");
+    MC_REQUIRE(m->grad_ready, "call mc_grad_init first");
+    MC_HIP(hipSetDevice(m->device));
+    const bool sh = t == m->T;
+    const int64_t n = sh ? m->K : m->n_qd;
+    for (int64_t i = 0; i < n; i++)
+        MC_REQUIRE(std::isfinite(logits[i]) && std::fabs(logits[i]) <= kLogitMax, "logit out of [-13.8, 13.8]");
+    float* dl = sh ? m->d_sh_logit : m->d_logit[t];
+    float* dv = sh ? m->d_sh_val[m->cur] : m->d_val[m->cur][t];
+    MC_HIP(hipMemcpy(dl, logits, (size_t)n * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(sigmoid_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, dl, dv, n);
+    MC_HIP(hipGetLastError());
+    if (sh) {
+        hipLaunchKernelGGL(sigmoid_kernel, dim3(1), dim3(256), 0, m->stream, dl, m->d_sh_num[m->cur], n);
+        MC_TRY(fill(m, m->d_sh_den[m->cur], 1.f, n));
+    } else {
+        m->den_state[t] = 3;
+    }
+    MC_HIP(hipStreamSynchronize(m->stream));
+    return MC_OK;
+}
+
+}  // extern "C"

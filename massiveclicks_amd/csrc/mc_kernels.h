@@ -460,6 +460,62 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
     }
 }
 
+// --------------------------------------------------------------------------
+// Shared by the EM and the gradient kernels.
+// --------------------------------------------------------------------------
+
+// Number of per-qd tables, shared-table length and shared accumulators.
+template <int MODEL> struct Shape;
+template <> struct Shape<kPBM> { static constexpr int T = 1, K = 10, KACC = 10; };
+template <> struct Shape<kUBM> { static constexpr int T = 1, K = 55, KACC = 55; };
+template <> struct Shape<kDBN> { static constexpr int T = 2, K = 1, KACC = 2; };
+template <> struct Shape<kCCM> { static constexpr int T = 1, K = 3, KACC = 6; };
+
+// Buffer descriptors of the session stream: 32-bit offsets (no 64-bit
+// address arithmetic per load) and hardware bounds checks (a prefetch past
+// the last session reads zeros).
+struct Stream {
+    __amdgpu_buffer_rsrc_t pidx, mask, len;
+    uint32_t row_bytes;
+};
+
+__device__ __forceinline__ Stream make_stream(const KArgs& A) {
+    constexpr int kRawDwordFlags = 0x00020000;   // raw 32-bit buffer, no format conversion
+    Stream st;
+    st.pidx = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx, 0, (int)(uint32_t)(A.N * 40), kRawDwordFlags);
+    st.mask = __builtin_amdgcn_make_buffer_rsrc((void*)A.mask, 0, (int)(uint32_t)(A.N * 2), kRawDwordFlags);
+    st.len = A.len ? __builtin_amdgcn_make_buffer_rsrc((void*)A.len, 0, (int)(uint32_t)A.N, kRawDwordFlags) : st.mask;
+    st.row_bytes = (uint32_t)(A.N * 4);
+    return st;
+}
+
+// Loads one session's click bits, length and query-document indices
+// (rank-major layout: rank r of session s at pidx[r * N + s]).
+// Lanes past the end of their item (s >= s_end) re-read the item's last
+// session, whose cache lines are already fetched: no branch, no extra DRAM
+// traffic, and the values are never used.  s_end <= 0 (no item) reads past
+// the buffer, which returns zeros.
+template <bool VARLEN>
+__device__ __forceinline__ void load_session(const Stream& st, int s, int s_end, uint32_t& m, int& L, int (&pv)[kL]) {
+    const uint32_t o = (uint32_t)min(s, s_end - 1);
+    m = __builtin_amdgcn_raw_buffer_load_b16(st.mask, o * 2u, 0, 0);
+    L = VARLEN ? (int)__builtin_amdgcn_raw_buffer_load_b8(st.len, o, 0, 0) : kL;
+#pragma unroll
+    for (int r = 0; r < kL; r++) pv[r] = (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, o * 4u, r * st.row_bytes, 0);
+}
+
+// Wave-level sum of a 64-bit integer (wave64).
+__device__ __forceinline__ long long wave_sum(long long v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Fixed point (32.32) plus a float base, rounded to float.
+__device__ __forceinline__ float fix_to_float(float base, unsigned long long v) {
+    return (float)((double)base + (double)(long long)v * kFixInv);
+}
+
 }  // namespace mc
 
 #endif  // MC_KERNELS_H

@@ -20,7 +20,8 @@ PBM, CCM, DBN, UBM = 0, 1, 2, 3          # main.cpp:178-183
 MODEL_NAMES = {PBM: "PBM", CCM: "CCM", DBN: "DBN", UBM: "UBM"}
 MAX_SERP = 10
 
-OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK, OPT_DIRECT, OPT_PERSISTENT = range(7)
+(OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK, OPT_DIRECT, OPT_PERSISTENT,
+ OPT_BATCHES) = range(8)
 REDUCE_SEGMENTED, REDUCE_ATOMIC = 0, 1
 
 
@@ -94,7 +95,7 @@ class ClickModel:
     def __init__(self, model_type: int, device: int = 0, reduce_mode: int = REDUCE_SEGMENTED,
                  item_sessions: int | None = None, item_window: int | None = None,
                  sort_by_lcr: bool = True, block: int | None = None, direct: bool = False,
-                 persistent: bool = True):
+                 persistent: bool = True, batches: int = 1):
         self.model_type = int(model_type)
         self._h = C.c_void_p()
         L = _lib.lib()
@@ -103,6 +104,7 @@ class ClickModel:
         check(L.mc_model_set_option(self._h, OPT_SORT_BY_LCR, int(bool(sort_by_lcr))))
         check(L.mc_model_set_option(self._h, OPT_DIRECT, int(bool(direct))))
         check(L.mc_model_set_option(self._h, OPT_PERSISTENT, int(bool(persistent))))
+        check(L.mc_model_set_option(self._h, OPT_BATCHES, int(batches)))
         if item_sessions is not None:
             check(L.mc_model_set_option(self._h, OPT_ITEM_SESSIONS, int(item_sessions)))
         if item_window is not None:
@@ -162,6 +164,41 @@ class ClickModel:
 
     def reset_parameters(self):
         check(_lib.lib().mc_reset_parameters(self._h))
+
+    # -- gradient path ----------------------------------------------------------
+    def grad_init(self):
+        """Logits from the current values, Adam state to zero."""
+        check(_lib.lib().mc_grad_init(self._h))
+
+    def loglik_gradient(self, batch: int = -1):
+        """(training log-likelihood, [d LL / d logit per table]) at the
+        current values, over minibatch `batch` (-1 = every session)."""
+        info = self.get_parameter_information()
+        grads = [np.empty(size, np.float32) for _, _, size in info]
+        arr = (f32p * len(grads))(*[g.ctypes.data_as(f32p) for g in grads])
+        ll = C.c_double()
+        check(_lib.lib().mc_loglik_gradient(self._h, int(batch), C.byref(ll), arr))
+        return ll.value, grads
+
+    def adam_configure(self, lr=0.05, beta1=0.9, beta2=0.999, eps=1e-8, prior=True):
+        check(_lib.lib().mc_adam_configure(self._h, float(lr), float(beta1), float(beta2), float(eps), int(bool(prior))))
+
+    def adam_steps(self, n_steps: int, use_graph: bool = True):
+        check(_lib.lib().mc_adam_steps(self._h, int(n_steps), int(bool(use_graph))))
+
+    def adam_steps_timed(self, n_steps: int) -> float:
+        ms = C.c_double()
+        check(_lib.lib().mc_adam_steps_timed(self._h, int(n_steps), C.byref(ms)))
+        return ms.value
+
+    def get_logits(self, t: int):
+        out = np.empty(self.get_parameter_information()[t][2], np.float32)
+        check(_lib.lib().mc_get_logits(self._h, int(t), ptr(out, f32p)))
+        return out
+
+    def set_logits(self, t: int, logits):
+        x = np.ascontiguousarray(logits, np.float32)
+        check(_lib.lib().mc_set_logits(self._h, int(t), ptr(x, f32p)))
 
     # -- tables -----------------------------------------------------------------
     def n_tables(self) -> int:

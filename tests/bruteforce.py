@@ -163,3 +163,34 @@ def factor_posteriors_fresh_tail(model, clicks, a, s=None, g=None, taus=None):
             w[:, :, 0] *= t0
         out.append(w / w.sum())
     return np.array(out)
+
+
+def fisher_terms(model, clicks, a, s=None, g=None, taus=None):
+    """Per-rank E[X - p | clicks] for every Bernoulli variable X ~ Bern(p)
+    of the generative story, i.e. the gradient of log P(clicks) with
+    respect to each parameter's logit (Fisher's identity).  Returns
+    (d_attr[L], d_sat[L], d_shared) with d_shared = [g] (DBN) or
+    [t1, t2, t3] (CCM); for the CCM the satisfaction draws S_r ~ Bern(a_r)
+    count towards d_attr."""
+    C, A, E, S, p = cascade_joint(model, a=a, s=s, g=g, taus=taus)
+    L = len(a)
+    c = np.asarray(clicks)
+    m = np.all(C == c[None, :], axis=1)
+    w = p[m] / p[m].sum()
+    A, S, E, C = A[m], S[m], E[m], C[m]
+    H = _assignments(3 * L)[m]
+    G = H[:, 2 * L:3 * L]
+    d_attr = np.array([(w * (A[:, r] - a[r])).sum() for r in range(L)])
+    if model == "dbn":
+        d_sat = np.array([(w * (S[:, r] - s[r])).sum() for r in range(L)])
+        d_sh = np.array([sum((w * (G[:, r] - g)).sum() for r in range(L))])
+        return d_attr, d_sat, d_sh
+    t1, t2, t3 = taus
+    d_attr = d_attr + np.array([(w * (S[:, r] - a[r])).sum() for r in range(L)])
+    d_sh = np.zeros(3)
+    for r in range(L):
+        clicked = C[:, r] == 1
+        d_sh[0] += (w * np.where(~clicked, G[:, r] - t1, 0)).sum()
+        d_sh[1] += (w * np.where(clicked & (S[:, r] == 0), G[:, r] - t2, 0)).sum()
+        d_sh[2] += (w * np.where(clicked & (S[:, r] == 1), G[:, r] - t3, 0)).sum()
+    return d_attr, np.zeros(L), d_sh

@@ -369,3 +369,50 @@ def test_query_aligned_threads_give_identical_per_qd_tables(model):
     # The merge order of FP32 partials alone moves the shared table by about
     # 1.5e-5 relative at 2,400 sessions: the reference's own run-to-run spread.
     np.testing.assert_allclose(outs[0][-1][2], outs[1][-1][2], rtol=1e-4)
+
+
+@pytest.mark.parametrize("name", ["dbn", "ccm"])
+def test_fisher_terms_helper_is_the_logit_gradient(name):
+    """The enumeration helper used to check the GPU gradient: its expected
+    counts minus expectations equal float64 central differences of
+    log P(clicks) with respect to each logit."""
+    from bruteforce import cascade_joint, fisher_terms, pattern_prob
+    rng = np.random.default_rng(9)
+    L = 4
+    a = rng.uniform(0.1, 0.9, L)
+    s = rng.uniform(0.1, 0.9, L)
+    g = 0.7
+    taus = np.array([0.8, 0.5, 0.3])
+    clicks = np.array([0, 1, 0, 1])
+
+    def logp(a, s, g, taus):
+        C, _, _, _, p = cascade_joint(name, a=a, s=s, g=g, taus=taus)
+        return np.log(pattern_prob(C, p, clicks))
+
+    def logit_shift(v, d):
+        x = np.log(v / (1 - v)) + d
+        return 1 / (1 + np.exp(-x))
+
+    da, dsat, dsh = fisher_terms(name, clicks, a=a, s=s, g=g, taus=taus)
+    eps = 1e-5
+    for r in range(L):
+        for arr, want in ((a, da[r]), (s, dsat[r])):
+            hi, lo = arr.copy(), arr.copy()
+            hi[r] = logit_shift(arr[r], eps)
+            lo[r] = logit_shift(arr[r], -eps)
+            args_hi = (hi, s, g, taus) if arr is a else (a, hi, g, taus)
+            args_lo = (lo, s, g, taus) if arr is a else (a, lo, g, taus)
+            fd = (logp(*args_hi) - logp(*args_lo)) / (2 * eps)
+            if name == "ccm" and arr is s:
+                continue
+            assert fd == pytest.approx(want, abs=1e-6), (r, arr is a)
+    if name == "dbn":
+        fd = (logp(a, s, logit_shift(g, eps), taus) - logp(a, s, logit_shift(g, -eps), taus)) / (2 * eps)
+        assert fd == pytest.approx(dsh[0], abs=1e-6)
+    else:
+        for k in range(3):
+            hi, lo = taus.copy(), taus.copy()
+            hi[k] = logit_shift(taus[k], eps)
+            lo[k] = logit_shift(taus[k], -eps)
+            fd = (logp(a, s, g, hi) - logp(a, s, g, lo)) / (2 * eps)
+            assert fd == pytest.approx(dsh[k], abs=1e-6), k
