@@ -10,6 +10,10 @@ HBM.  Prints one JSON line:
               against the 8 TB/s HBM peak
   cpu_baseline  the CPU oracle (reference-faithful threading) on a bounded
               sample of the same workload, on this host's cores
+  time_to_target  DBN on 20M sessions: wall time of plain EM's 50
+              iterations, and of Adam on the log-likelihood gradient
+              (lr 0.05, minibatches of ~1M sessions) to reach EM's
+              50-iteration held-out log-likelihood within 1e-4 relative
 
 Algorithmic bytes (per iteration): 42 B per training session (ten int32
 query-document indices + a 16-bit click mask) + 16 B per table entry and
@@ -65,6 +69,59 @@ def cpu_baseline(ds, model, n_sample, iters, threads):
                       f"{threads} threads: {dt:.3f} s/iter (E {te / iters:.3f} s, M {tm / iters:.3f} s)"}
 
 
+def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
+    """Wall time for plain EM (50 iterations) and for Adam on the
+    log-likelihood gradient to reach the held-out log-likelihood of EM's
+    50th iteration (within 1e-4 relative), DBN.  EM iterations are replayed
+    from one graph and timed with HIP events; Adam epochs (every minibatch
+    once) likewise; held-out evaluations between epochs are not timed."""
+    from massiveclicks_amd import Dataset, ClickModel, synth
+    from massiveclicks_amd.model import DBN
+    t0 = time.perf_counter()
+    s = synth.generate(synth.SynthConfig(n_sessions=n_sessions, n_queries=1_000_000, vocab=20_000_000, zipf_s=1.1,
+                                         model="dbn", seed=seed))
+    ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2)
+    del s
+    log(f"time-to-target data: train {ds.n_train}, test {ds.n_test}, n_qd {ds.n_qd} in {time.perf_counter() - t0:.1f} s")
+    test = ds.test
+    with ClickModel(DBN) as cm:
+        cm.init_from_dataset(ds)
+        em_ms = cm.em_graph_timed(50)      # one warm replay of 50 iterations from the start, then 50 timed
+        cm.reset_parameters()
+        cm.em(50)
+        target = cm.evaluate(test)["log_likelihood"]
+    thr = target - 1e-4 * abs(target)
+    batches = max(1, round(ds.n_train / batch_sessions))
+    curve = []
+    reached = None
+    with ClickModel(DBN, batches=batches) as cm:
+        cm.init_from_dataset(ds)
+        cm.grad_init()
+        cm.adam_configure(lr=lr)
+        total = 0.0
+        for ep in range(1, max_epochs + 1):
+            total += cm.adam_steps_timed(batches)
+            ll = cm.evaluate(test)["log_likelihood"]
+            curve.append([ep, round(total, 3), round(ll, 6)])
+            if ll >= thr:
+                reached = (ep, total, ll)
+                break
+    log(f"time-to-target: EM 50 it {em_ms:.2f} ms -> LL {target:.6f}; Adam {curve[-1]}")
+    return {
+        "workload": f"DBN, {n_sessions} synthetic sessions ({ds.n_train} train), 1M Zipf(1.1) queries; Adam lr {lr}, "
+                    f"{batches} minibatches of whole query groups (~{batch_sessions} sessions), log-prior on",
+        "target_heldout_ll": target,
+        "threshold": "held-out LL within 1e-4 relative of EM's 50-iteration LL",
+        "em_ms": em_ms,
+        "adam_ms": reached[1] if reached else None,
+        "adam_epochs": reached[0] if reached else None,
+        "adam_steps": reached[0] * batches if reached else None,
+        "adam_ll": reached[2] if reached else curve[-1][2],
+        "speedup_vs_em": (em_ms / reached[1]) if reached else None,
+        "adam_curve": curve,
+    }
+
+
 def calibration_copy(mib):
     """One device-to-device copy of `mib` MiB (a plain copy kernel of known
     size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -95,6 +152,10 @@ def main():
     ap.add_argument("--item-sessions", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--calib-mib", type=int, default=0, help="also run a D2D copy of this size")
+    ap.add_argument("--ttt-sessions", type=int, default=20_000_000, help="time-to-target workload (0 = skip)")
+    ap.add_argument("--ttt-batch", type=int, default=1_000_000)
+    ap.add_argument("--ttt-lr", type=float, default=0.05)
+    ap.add_argument("--ttt-max-epochs", type=int, default=40)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -190,6 +251,9 @@ def main():
         "heldout": {"log_likelihood": ev["log_likelihood"], "perplexity": ev["perplexity"]},
     }
     cm.close()
+    del ds
+    if args.ttt_sessions and args.model == "dbn":
+        out["time_to_target"] = time_to_target(args.ttt_sessions, args.ttt_batch, args.ttt_lr, args.ttt_max_epochs)
     if args.calib_mib:
         calibration_copy(args.calib_mib)
     print(json.dumps(out), flush=True)

@@ -63,6 +63,7 @@ struct GArgs {
     double* ll_part;               // per item
     const int32_t* shared_idx;
     int32_t* steps;                // Adam steps per batch, then of the shared tables
+    unsigned* merge_done;          // blocks of the current merge that have finished
     int32_t K, kacc, max_width, batch, n_batches, use_prior;
     float lr, b1, b2, eps;
     double sh_scale;               // training sessions / batch sessions
@@ -106,7 +107,7 @@ __device__ __forceinline__ float sigmoid(float x) { return 1.f / (1.f + __expf(-
 
 // PBM/UBM: log P(C) = sum_r c log(a e) + (1 - c) log(1 - a e), e from
 // rank r (PBM) or slot r(r+1)/2 + prev (UBM, ubm.cu:201).
-template <int MODEL, bool VARLEN, class Src, class Emit>
+template <int MODEL, bool VARLEN, bool LL, class Src, class Emit>
 __device__ __forceinline__ void grad_pbm_ubm(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                              const float* sh, float (&acc)[55], float& ll) {
     int slot[kL];
@@ -120,11 +121,11 @@ __device__ __forceinline__ void grad_pbm_ubm(const int (&pv)[kL], int L, uint32_
         const float ae = a * e;
         float ge;
         if ((mm >> r) & 1u) {
-            ll += __logf(ae);
+            if (LL) ll += __logf(ae);
             ge = 1.f - e;
         } else {
             const float q = 1.f - ae;
-            ll += __logf(q);
+            if (LL) ll += __logf(q);
             const float w = ae * __builtin_amdgcn_rcpf(q);
             em.add(0, r, p, -w * (1.f - a));
             ge = -w * (1.f - e);
@@ -148,7 +149,7 @@ __device__ __forceinline__ void grad_pbm_ubm(const int (&pv)[kL], int L, uint32_
 //             d/dg += dex (1-s), d LL/d ex_r = 1/ex_r;
 //   no click  q = 1 - a ex: d/da = -ex/q + dex ex g (ex-1)/q^2,
 //             d/dg += dex ex (1-a)/q, d LL/d ex_r = -a/q + dex g (1-a)/q^2.
-template <int MODEL, bool VARLEN, class Src, class Emit>
+template <int MODEL, bool VARLEN, bool LL, class Src, class Emit>
 __device__ __forceinline__ void grad_cascade(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                              const float* sh, float (&acc)[55], float& ll) {
 #pragma clang fp contract(fast)
@@ -162,11 +163,11 @@ __device__ __forceinline__ void grad_cascade(const int (&pv)[kL], int L, uint32_
         if (VARLEN && r >= L) continue;
         const float a = src.get(0, pv[r]);
         if ((mm >> r) & 1u) {
-            ll += __logf(a * e);
+            if (LL) ll += __logf(a * e);
             e = MODEL == kDBN ? g * (1.f - src.get(1, pv[r])) : t2 * (1.f - a) + t3 * a;
         } else {
             const float q = 1.f - a * e;
-            ll += __logf(q);
+            if (LL) ll += __logf(q);
             e = e * g * (1.f - a) * __builtin_amdgcn_rcpf(q);
         }
     }
@@ -205,12 +206,12 @@ __device__ __forceinline__ void grad_cascade(const int (&pv)[kL], int L, uint32_
     }
 }
 
-template <int MODEL, bool VARLEN, class Src, class Emit>
+template <int MODEL, bool VARLEN, bool LL, class Src, class Emit>
 __device__ __forceinline__ void grad_session(const int (&pv)[kL], int L, uint32_t m, const Src& src, Emit& em,
                                              const float* sh, float (&acc)[55], float& ll) {
     const uint32_t mm = m & ((1u << L) - 1u);
-    if (MODEL == kPBM || MODEL == kUBM) grad_pbm_ubm<MODEL, VARLEN>(pv, L, mm, src, em, sh, acc, ll);
-    else grad_cascade<MODEL, VARLEN>(pv, L, mm, src, em, sh, acc, ll);
+    if (MODEL == kPBM || MODEL == kUBM) grad_pbm_ubm<MODEL, VARLEN, LL>(pv, L, mm, src, em, sh, acc, ll);
+    else grad_cascade<MODEL, VARLEN, LL>(pv, L, mm, src, em, sh, acc, ll);
 }
 
 // Adam (maximising): m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
@@ -262,19 +263,19 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // item's LDS window (WIN) or with global fixed-point atomics, then
 // finalised for the entries the item owns: written out (ADAM = false) or
 // applied by Adam in place.
-template <int MODEL, bool VARLEN, bool WIN, bool ADAM>
-__global__ void __launch_bounds__(256) grad_item_kernel(GArgs G, int64_t list_off) {
+template <int MODEL, bool VARLEN, bool WIN, bool ADAM, int NT>
+__global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off) {
     constexpr int T = Shape<MODEL>::T;
     constexpr int K = Shape<MODEL>::K;
+    constexpr int NW = NT / 64;
     extern __shared__ unsigned long long lds[];
     __shared__ float sh[K < 64 ? 64 : K];
-    __shared__ long long red[K][4];
-    __shared__ double lred[4];
+    __shared__ long long red[K][NW];
+    __shared__ double lred[NW];
 
     const int item_idx = G.lists[list_off + blockIdx.x];
     const Item it = G.items[item_idx];
     const int tid = threadIdx.x;
-    constexpr int NT = 256;
     const int W = WIN ? it.width : 0;
     unsigned long long* lacc = lds;
     float* lval = (float*)(lds + T * G.max_width);
@@ -315,8 +316,8 @@ __global__ void __launch_bounds__(256) grad_item_kernel(GArgs G, int64_t list_of
             int L2;
             int pv2[kL];
             load_session<VARLEN>(st, s + NT, it.s1, m2, L2, pv2);
-            if (WIN) grad_session<MODEL, VARLEN>(pv0, L0, m0, lsrc, el, sh, acc, ll);
-            else grad_session<MODEL, VARLEN>(pv0, L0, m0, gsrc, eg, sh, acc, ll);
+            if (WIN) grad_session<MODEL, VARLEN, !ADAM>(pv0, L0, m0, lsrc, el, sh, acc, ll);
+            else grad_session<MODEL, VARLEN, !ADAM>(pv0, L0, m0, gsrc, eg, sh, acc, ll);
             m0 = m2;
             L0 = L2;
 #pragma unroll
@@ -332,14 +333,23 @@ __global__ void __launch_bounds__(256) grad_item_kernel(GArgs G, int64_t list_of
         const long long v = wave_sum(to_fix_signed(acc[k]));
         if (lane == 0) red[k][wave] = v;
     }
-    const double lw = wave_sum_f64((double)ll);
-    if (lane == 0) lred[wave] = lw;
+    if (!ADAM) {
+        const double lw = wave_sum_f64((double)ll);
+        if (lane == 0) lred[wave] = lw;
+    }
     __syncthreads();
     for (int k = tid; k < K; k += NT) {
-        const long long v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        long long v = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) v += red[k][w];
         if (v != 0) atomicAdd(&G.sh_acc[k], (unsigned long long)v);
     }
-    if (!ADAM && tid == 0) G.ll_part[item_idx] = ((lred[0] + lred[1]) + lred[2]) + lred[3];
+    if (!ADAM && tid == 0) {
+        double lsum = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) lsum += lred[w];
+        G.ll_part[item_idx] = lsum;
+    }
 
     if (WIN) {
         const bool interior = (it.flags & kItemInterior) != 0;
@@ -367,7 +377,8 @@ __global__ void __launch_bounds__(256) grad_item_kernel(GArgs G, int64_t list_of
 }
 
 // Block 0: the shared tables (gradient scaled to the training set, prior,
-// Adam, step counters); other blocks: the listed split entries.
+// Adam); other blocks: the listed split entries.  With Adam, the last block
+// to finish advances the step counts, after every block has read them.
 template <int MODEL, bool ADAM>
 __global__ void __launch_bounds__(256) grad_merge_kernel(GArgs G, int64_t shared_off, int64_t shared_cnt) {
     constexpr int T = Shape<MODEL>::T;
@@ -390,21 +401,33 @@ __global__ void __launch_bounds__(256) grad_merge_kernel(GArgs G, int64_t shared
         }
         __syncthreads();
         for (int j = tid; j < G.kacc; j += blockDim.x) G.sh_acc[j] = 0ull;
-        return;
-    }
-    const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
-    if (i >= shared_cnt) return;
-    const int p = G.shared_idx[shared_off + i];
-    const int2 oc = G.counts[p];
-    AdamStep ad;
-    if (ADAM) ad = adam_step(G, G.steps[G.batch] + 1);
+    } else {
+        const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
+        if (i < shared_cnt) {
+            const int p = G.shared_idx[shared_off + i];
+            const int2 oc = G.counts[p];
+            AdamStep ad;
+            if (ADAM) ad = adam_step(G, G.steps[G.batch] + 1);
 #pragma unroll
-    for (int t = 0; t < T; t++) {
-        const float val = G.val[t][p];
-        const float g = entry_gradient<MODEL>(t, fix_to_double(G.gacc[t][p]), oc, val, ADAM && G.use_prior);
-        if (ADAM) G.val[t][p] = ad.apply(G.logit[t][p], G.am[t][p], G.av[t][p], g);
-        else G.grad[t][p] = g;
-        G.gacc[t][p] = 0ull;
+            for (int t = 0; t < T; t++) {
+                const float val = G.val[t][p];
+                const float g = entry_gradient<MODEL>(t, fix_to_double(G.gacc[t][p]), oc, val, ADAM && G.use_prior);
+                if (ADAM) G.val[t][p] = ad.apply(G.logit[t][p], G.am[t][p], G.av[t][p], g);
+                else G.grad[t][p] = g;
+                G.gacc[t][p] = 0ull;
+            }
+        }
+    }
+    if (ADAM) {
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            if (atomicAdd(G.merge_done, 1u) == gridDim.x - 1) {
+                atomicAdd(&G.steps[G.batch], 1);
+                atomicAdd(&G.steps[G.n_batches], 1);
+                atomicExch(G.merge_done, 0u);
+            }
+        }
     }
 }
 
@@ -421,15 +444,6 @@ __global__ void logit_init_kernel(const float* val, float* logit, float* am, flo
 __global__ void sigmoid_kernel(const float* logit, float* val, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) val[i] = sigmoid(logit[i]);
-}
-
-// Counts one Adam step of `batch` and of the shared tables; runs after the
-// step's kernels, which read the counts.
-__global__ void advance_steps_kernel(int32_t* steps, int batch, int n_batches) {
-    if (threadIdx.x == 0) {
-        steps[batch] += 1;
-        steps[n_batches] += 1;
-    }
 }
 
 __global__ void zero_steps_kernel(int32_t* steps, int n) {
@@ -466,7 +480,8 @@ mc_status ensure_grad_buffers(mc_model* m) {
     MC_TRY(dalloc(m, &m->d_sh_grad, (size_t)K * 4));
     MC_TRY(dalloc(m, &m->d_sh_prior, (size_t)K * 8));
     MC_TRY(dalloc(m, &m->d_ll_part, (size_t)m->n_items * 8));
-    MC_TRY(dalloc(m, &m->d_steps, (size_t)(m->n_batches + 1) * 4));
+    MC_TRY(dalloc(m, &m->d_steps, (size_t)(m->n_batches + 2) * 4));   // counts, then the merge's finished-block count
+    MC_HIP(hipMemset(m->d_steps, 0, (size_t)(m->n_batches + 2) * 4));
     // Prior weights of the shared tables: 1 + the sessions that leave the
     // slot's temporary at (1,2) (the phantom count in sh_base).
     std::vector<double> base(2 * (size_t)K), prior((size_t)K);
@@ -480,6 +495,7 @@ mc_status ensure_grad_buffers(mc_model* m) {
         prior[(size_t)k] = 1.0 + ph;
     }
     MC_HIP(hipMemcpy(m->d_sh_prior, prior.data(), prior.size() * 8, hipMemcpyHostToDevice));
+    MC_HIP(hipDeviceGetAttribute(&m->n_cu, hipDeviceAttributeMultiprocessorCount, m->device));
     m->grad_ready = true;
     return MC_OK;
 }
@@ -515,6 +531,7 @@ GArgs make_gargs(mc_model* m, int batch) {
     G.ll_part = m->d_ll_part;
     G.shared_idx = m->d_shared_idx;
     G.steps = m->d_steps;
+    G.merge_done = (unsigned*)(m->d_steps + m->n_batches + 1);
     G.K = m->K;
     G.kacc = m->KACC;
     G.max_width = m->max_width;
@@ -530,30 +547,33 @@ GArgs make_gargs(mc_model* m, int batch) {
     return G;
 }
 
+template <int MODEL, bool WIN, bool ADAM, int NT>
+void launch_items(mc_model* m, const GArgs& G, int64_t count, int64_t off, size_t lds) {
+    if (m->varlen)
+        hipLaunchKernelGGL((grad_item_kernel<MODEL, true, WIN, ADAM, NT>), dim3((unsigned)count), dim3(NT), lds,
+                           m->stream, G, off);
+    else
+        hipLaunchKernelGGL((grad_item_kernel<MODEL, false, WIN, ADAM, NT>), dim3((unsigned)count), dim3(NT), lds,
+                           m->stream, G, off);
+}
+
 template <int MODEL, bool ADAM>
 void launch_grad_t(mc_model* m, const GArgs& G, int batch) {
     const size_t b = batch < 0 ? (size_t)m->n_batches : (size_t)batch;
     const int64_t nw = m->win_cnt[b], ng = m->glob_cnt[b];
+    // Few items (a small minibatch): 1024 lanes per item to fill the chip.
+    const bool wide = nw + ng < 4 * (int64_t)m->n_cu;
     if (nw > 0) {
-        if (m->varlen)
-            hipLaunchKernelGGL((grad_item_kernel<MODEL, true, true, ADAM>), dim3((unsigned)nw), dim3(256), m->lds_bytes,
-                               m->stream, G, m->win_off[b]);
-        else
-            hipLaunchKernelGGL((grad_item_kernel<MODEL, false, true, ADAM>), dim3((unsigned)nw), dim3(256),
-                               m->lds_bytes, m->stream, G, m->win_off[b]);
+        if (wide) launch_items<MODEL, true, ADAM, 1024>(m, G, nw, m->win_off[b], m->lds_bytes);
+        else launch_items<MODEL, true, ADAM, 256>(m, G, nw, m->win_off[b], m->lds_bytes);
     }
     if (ng > 0) {
-        if (m->varlen)
-            hipLaunchKernelGGL((grad_item_kernel<MODEL, true, false, ADAM>), dim3((unsigned)ng), dim3(256), 0, m->stream,
-                               G, m->glob_off[b]);
-        else
-            hipLaunchKernelGGL((grad_item_kernel<MODEL, false, false, ADAM>), dim3((unsigned)ng), dim3(256), 0,
-                               m->stream, G, m->glob_off[b]);
+        if (wide) launch_items<MODEL, false, ADAM, 1024>(m, G, ng, m->glob_off[b], 0);
+        else launch_items<MODEL, false, ADAM, 256>(m, G, ng, m->glob_off[b], 0);
     }
     const int64_t so = m->shared_off[b], sc = m->shared_cnt[b];
     const unsigned grid = (unsigned)(1 + (sc + 255) / 256);
     hipLaunchKernelGGL((grad_merge_kernel<MODEL, ADAM>), dim3(grid), dim3(256), 0, m->stream, G, so, sc);
-    if (ADAM) hipLaunchKernelGGL(advance_steps_kernel, dim3(1), dim3(64), 0, m->stream, G.steps, G.batch, G.n_batches);
 }
 
 mc_status launch_grad(mc_model* m, int batch, bool adam) {

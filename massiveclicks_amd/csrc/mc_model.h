@@ -104,6 +104,7 @@ struct mc_model {
     double* d_sh_prior = nullptr;              // per shared entry: pseudo-count weight of the prior
     double* d_ll_part = nullptr;               // per item: log-likelihood of its sessions
     int32_t* d_steps = nullptr;                // Adam step count per batch, then of the shared tables
+    int n_cu = 0;
     float adam_lr = 0.05f, adam_b1 = 0.9f, adam_b2 = 0.999f, adam_eps = 1e-8f;
     int adam_prior = 1;
     int adam_next_batch = 0;
