@@ -64,6 +64,7 @@ struct GArgs {
     const int32_t* shared_idx;
     int32_t* steps;                // Adam steps per batch, then of the shared tables
     unsigned* merge_done;          // blocks of the current merge that have finished
+    int64_t shared_off, shared_cnt;  // split entries of the batch
     int32_t K, kacc, max_width, batch, n_batches, use_prior;
     float lr, b1, b2, eps;
     double sh_scale;               // training sessions / batch sessions
@@ -259,6 +260,63 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return v;
 }
 
+// Merge work of grad_merge_kernel.
+__device__ __forceinline__ unsigned long long load_sum(const unsigned long long* p) { return *p; }
+
+template <int MODEL, bool ADAM>
+__device__ __forceinline__ void merge_shared_entry(const GArgs& G, int k) {
+    const double sum = fix_to_double(load_sum(&G.sh_acc[k]));
+    if (ADAM) {
+        const float val = G.sh_val[k];
+        double g = sum * G.sh_scale;
+        if (G.use_prior) g += G.sh_prior[k] * (1.0 - 2.0 * (double)val);
+        const AdamStep ad = adam_step(G, G.steps[G.n_batches] + 1);
+        const float nv = ad.apply(G.sh_logit[k], G.sh_am[k], G.sh_av[k], (float)g);
+        G.sh_val[k] = nv;
+        G.sh_num[k] = nv;
+        G.sh_den[k] = 1.f;
+    } else {
+        G.sh_grad[k] = (float)sum;
+    }
+}
+
+template <int MODEL, bool ADAM>
+__device__ __forceinline__ void merge_split_entry(const GArgs& G, int64_t i, const AdamStep& ad) {
+    constexpr int T = Shape<MODEL>::T;
+    const int p = G.shared_idx[G.shared_off + i];
+    const int2 oc = G.counts[p];
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const float val = G.val[t][p];
+        const float g = entry_gradient<MODEL>(t, fix_to_double(load_sum(&G.gacc[t][p])), oc, val, ADAM && G.use_prior);
+        if (ADAM) G.val[t][p] = ad.apply(G.logit[t][p], G.am[t][p], G.av[t][p], g);
+        else G.grad[t][p] = g;
+        G.gacc[t][p] = 0ull;
+    }
+}
+
+// The last workgroup of a launch to arrive (counted in merge_done) returns
+// true, after every other workgroup has finished its reads and atomics.
+__device__ __forceinline__ bool last_block(const GArgs& G) {
+    __shared__ int is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        is_last = atomicAdd(G.merge_done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    return is_last != 0;
+}
+
+// Step counts and the arrival count, by one thread of the last workgroup.
+__device__ __forceinline__ void finish_step(const GArgs& G, bool adam) {
+    if (adam) {
+        atomicAdd(&G.steps[G.batch], 1);
+        atomicAdd(&G.steps[G.n_batches], 1);
+    }
+    atomicExch(G.merge_done, 0u);
+}
+
 // One workgroup per listed item: the sessions' gradients, reduced in the
 // item's LDS window (WIN) or with global fixed-point atomics, then
 // finalised for the entries the item owns: written out (ADAM = false) or
@@ -380,55 +438,19 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
 // Adam); other blocks: the listed split entries.  With Adam, the last block
 // to finish advances the step counts, after every block has read them.
 template <int MODEL, bool ADAM>
-__global__ void __launch_bounds__(256) grad_merge_kernel(GArgs G, int64_t shared_off, int64_t shared_cnt) {
-    constexpr int T = Shape<MODEL>::T;
+__global__ void __launch_bounds__(256) grad_merge_kernel(GArgs G) {
     const int tid = threadIdx.x;
     if (blockIdx.x == 0) {
-        if (tid < G.K) {
-            const double sum = fix_to_double(G.sh_acc[tid]);
-            if (ADAM) {
-                const float val = G.sh_val[tid];
-                double g = sum * G.sh_scale;
-                if (G.use_prior) g += G.sh_prior[tid] * (1.0 - 2.0 * (double)val);
-                const AdamStep ad = adam_step(G, G.steps[G.n_batches] + 1);
-                const float nv = ad.apply(G.sh_logit[tid], G.sh_am[tid], G.sh_av[tid], (float)g);
-                G.sh_val[tid] = nv;
-                G.sh_num[tid] = nv;
-                G.sh_den[tid] = 1.f;
-            } else {
-                G.sh_grad[tid] = (float)sum;
-            }
-        }
+        if (tid < G.K) merge_shared_entry<MODEL, ADAM>(G, tid);
         __syncthreads();
         for (int j = tid; j < G.kacc; j += blockDim.x) G.sh_acc[j] = 0ull;
     } else {
         const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
-        if (i < shared_cnt) {
-            const int p = G.shared_idx[shared_off + i];
-            const int2 oc = G.counts[p];
-            AdamStep ad;
-            if (ADAM) ad = adam_step(G, G.steps[G.batch] + 1);
-#pragma unroll
-            for (int t = 0; t < T; t++) {
-                const float val = G.val[t][p];
-                const float g = entry_gradient<MODEL>(t, fix_to_double(G.gacc[t][p]), oc, val, ADAM && G.use_prior);
-                if (ADAM) G.val[t][p] = ad.apply(G.logit[t][p], G.am[t][p], G.av[t][p], g);
-                else G.grad[t][p] = g;
-                G.gacc[t][p] = 0ull;
-            }
-        }
+        AdamStep ad;
+        if (ADAM) ad = adam_step(G, G.steps[G.batch] + 1);
+        if (i < G.shared_cnt) merge_split_entry<MODEL, ADAM>(G, i, ad);
     }
-    if (ADAM) {
-        __syncthreads();
-        if (tid == 0) {
-            __threadfence();
-            if (atomicAdd(G.merge_done, 1u) == gridDim.x - 1) {
-                atomicAdd(&G.steps[G.batch], 1);
-                atomicAdd(&G.steps[G.n_batches], 1);
-                atomicExch(G.merge_done, 0u);
-            }
-        }
-    }
+    if (last_block(G) && tid == 0) finish_step(G, ADAM);
 }
 
 // logit = log(v / (1 - v)) of every entry; moments to zero.
@@ -558,9 +580,13 @@ void launch_items(mc_model* m, const GArgs& G, int64_t count, int64_t off, size_
 }
 
 template <int MODEL, bool ADAM>
-void launch_grad_t(mc_model* m, const GArgs& G, int batch) {
+void launch_grad_t(mc_model* m, const GArgs& G0, int batch) {
     const size_t b = batch < 0 ? (size_t)m->n_batches : (size_t)batch;
     const int64_t nw = m->win_cnt[b], ng = m->glob_cnt[b];
+    GArgs Gb = G0;
+    Gb.shared_off = m->shared_off[b];
+    Gb.shared_cnt = m->shared_cnt[b];
+    const GArgs& G = Gb;
     // Few items (a small minibatch): 1024 lanes per item to fill the chip.
     const bool wide = nw + ng < 4 * (int64_t)m->n_cu;
     if (nw > 0) {
@@ -571,9 +597,8 @@ void launch_grad_t(mc_model* m, const GArgs& G, int batch) {
         if (wide) launch_items<MODEL, false, ADAM, 1024>(m, G, ng, m->glob_off[b], 0);
         else launch_items<MODEL, false, ADAM, 256>(m, G, ng, m->glob_off[b], 0);
     }
-    const int64_t so = m->shared_off[b], sc = m->shared_cnt[b];
-    const unsigned grid = (unsigned)(1 + (sc + 255) / 256);
-    hipLaunchKernelGGL((grad_merge_kernel<MODEL, ADAM>), dim3(grid), dim3(256), 0, m->stream, G, so, sc);
+    const unsigned grid = (unsigned)(1 + (Gb.shared_cnt + 255) / 256);
+    hipLaunchKernelGGL((grad_merge_kernel<MODEL, ADAM>), dim3(grid), dim3(256), 0, m->stream, Gb);
 }
 
 mc_status launch_grad(mc_model* m, int batch, bool adam) {
