@@ -197,6 +197,23 @@ mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, s
 mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
                              int64_t* max_item_sessions);
 
+/* Number of mc_model_set_option options (indices 0 .. MC_N_OPTIONS - 1). */
+#define MC_N_OPTIONS 8
+
+/* Autotuner of the EM work decomposition; no counterpart in the reference
+ * (whose launch shape is fixed: BLOCK_SIZE 128, one session per thread).
+ * options[MC_N_OPTIONS] holds the starting option values on entry (a
+ * candidate itself) and the fastest found on exit.  Each candidate is
+ * measured as `iterations` EM iterations replayed from a graph on `device`
+ * on this training set (arguments as mc_init_parameters): the reduction mode
+ * first, then sessions per work item (2048..16384, i.e. sessions per lane),
+ * the LDS window and the launch shape (persistent kernel, or one workgroup
+ * of 256 or 512 lanes per item).  *best_ms_per_iter and *n_trials report the
+ * winner's time and the number of candidates measured. */
+mc_status mc_autotune(int model_type, int device, int64_t n, const int32_t* query, const int32_t* pidx,
+                      const uint8_t* clicks, const uint8_t* len, int64_t n_qd, int iterations, int64_t* options,
+                      double* best_ms_per_iter, int* n_trials);
+
 /* ---------------------------------------------------------------------------
  * Gradient path: direct maximum-likelihood fitting beside plain EM.  No
  * counterpart in the reference, which only runs EM.  The objective is the

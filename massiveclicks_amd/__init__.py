@@ -5,5 +5,5 @@ C ABI in include/mclicks.h); this package is its Python host interface.
 """
 from .model import (  # noqa: F401
     PBM, CCM, DBN, UBM, MODEL_NAMES, MAX_SERP, REDUCE_SEGMENTED, REDUCE_ATOMIC,
-    ClickModel, Dataset, SessionSet, em_train,
+    ClickModel, Dataset, SessionSet, autotune, em_train,
 )

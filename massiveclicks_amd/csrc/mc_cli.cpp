@@ -45,6 +45,7 @@ struct Config {
     int reduce_mode = MC_REDUCE_SEGMENTED;
     double test_share = 0.2;
     bool help = false;
+    bool autotune = false;
 };
 
 void show_help() {
@@ -62,6 +63,7 @@ void show_help() {
         "  -j, --job-id <id>          job id (reported)\n"
         "  -e, --exec-mode <id>       0 GPU-only (this build)\n"
         "      --reduce-mode <id>     0 deterministic segmented (default), 1 float atomics\n"
+        "      --autotune <0|1>       measure and pick the EM launch options first\n"
         "      --resume <file>        load a parameter checkpoint before training\n"
         "      --checkpoint <file>    save a parameter checkpoint after training\n"
         "  -h, --help                 this message\n";
@@ -93,6 +95,7 @@ int main(int argc, char** argv) {
         {"--job-id", "-j", [&](const char* v) { config.job_id = std::stoi(v); }},
         {"--exec-mode", "-e", [&](const char* v) { config.exec_mode = std::stoi(v); }},
         {"--reduce-mode", "", [&](const char* v) { config.reduce_mode = std::stoi(v); }},
+        {"--autotune", "", [&](const char* v) { config.autotune = std::stoi(v) != 0; }},
         {"--resume", "", [&](const char* v) { config.checkpoint_in = v; }},
         {"--checkpoint", "", [&](const char* v) { config.checkpoint_out = v; }},
         {"--help", "-h", [&](const char*) { config.help = true; }},
@@ -193,8 +196,20 @@ int main(int argc, char** argv) {
 
         // ---- EM (parallel_em.cu:349-609) ------------------------------------
         const auto t_em = std::chrono::steady_clock::now();
+        int64_t opts[MC_N_OPTIONS] = {config.reduce_mode, 8192, 1280, 1, 256, 0, 1, 1};
+        if (config.autotune) {
+            const mclicks::Sessions tr = ds.train();
+            double ms = 0.0;
+            int trials = 0;
+            mclicks::check(mc_autotune(config.model_type, 0, tr.size(), tr.query.data(), tr.pidx.data(),
+                                       tr.clicks.data(), nullptr, ds.n_qd(), 5, opts, &ms, &trials));
+            std::cout << "\nAutotune (" << trials << " candidates): reduce-mode " << opts[MC_OPT_REDUCE_MODE]
+                      << ", item-sessions " << opts[MC_OPT_ITEM_SESSIONS] << ", window " << opts[MC_OPT_ITEM_WINDOW]
+                      << ", persistent " << opts[MC_OPT_PERSISTENT] << ", block " << opts[MC_OPT_BLOCK] << ": " << ms
+                      << " ms/iteration" << std::endl;
+        }
         mclicks::ClickModel cm(config.model_type, 0);
-        cm.set_option(MC_OPT_REDUCE_MODE, config.reduce_mode);
+        for (int o = 0; o < MC_N_OPTIONS; o++) cm.set_option(o, opts[o]);
         const auto t_h2d = std::chrono::steady_clock::now();
         cm.init_parameters(ds);
         const double h2d_s = seconds_since(t_h2d);

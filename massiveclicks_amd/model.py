@@ -286,6 +286,53 @@ class ClickModel:
         return {"n_items": a.value, "n_shared_entries": b.value, "max_item_sessions": c.value}
 
 
+OPTION_NAMES = ("reduce_mode", "item_sessions", "item_window", "sort_by_lcr", "block", "direct", "persistent",
+                "batches")
+DEFAULT_OPTIONS = {"reduce_mode": REDUCE_SEGMENTED, "item_sessions": 8192, "item_window": 1280, "sort_by_lcr": 1,
+                   "block": 256, "direct": 0, "persistent": 1, "batches": 1}
+
+
+def autotune(model_type: int, train: SessionSet, n_qd: int, device: int = 0, iterations: int = 5,
+             start: dict | None = None, cache: str | None = None) -> dict:
+    """Fastest EM options for this model and training set (mc_autotune).
+    Returns (options, info): options are keyword arguments of ClickModel,
+    info = {ms_per_iter, trials, cached}.  With `cache` (a JSON file) results
+    are stored per (model, sessions, n_qd, uneven lengths) and reused."""
+    import json
+    import os
+    key = f"{MODEL_NAMES[model_type]}:{len(train)}:{n_qd}:{int(train.length is not None)}"
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            known = json.load(f)
+        if key in known:
+            return dict(known[key]["options"]), {"ms_per_iter": known[key]["ms_per_iter"],
+                                                 "trials": known[key]["trials"], "cached": True}
+    opts = dict(DEFAULT_OPTIONS, **(start or {}))
+    arr = np.array([int(opts[k]) for k in OPTION_NAMES], np.int64)
+    q = np.ascontiguousarray(train.query, np.int32)
+    p = np.ascontiguousarray(train.pidx, np.int32)
+    c = np.ascontiguousarray(train.clicks, np.uint8)
+    ln = None if train.length is None else np.ascontiguousarray(train.length, np.uint8)
+    ms = C.c_double()
+    n_trials = C.c_int()
+    check(_lib.lib().mc_autotune(int(model_type), int(device), len(q), ptr(q, i32p), ptr(p, i32p), ptr(c, u8p),
+                                 ptr(ln, u8p), int(n_qd), int(iterations), ptr(arr, i64p), C.byref(ms),
+                                 C.byref(n_trials)))
+    best = {k: int(v) for k, v in zip(OPTION_NAMES, arr)}
+    best["sort_by_lcr"] = bool(best["sort_by_lcr"])
+    best["direct"] = bool(best["direct"])
+    best["persistent"] = bool(best["persistent"])
+    if cache:
+        known = {}
+        if os.path.exists(cache):
+            with open(cache) as f:
+                known = json.load(f)
+        known[key] = {"options": best, "ms_per_iter": ms.value, "trials": n_trials.value}
+        with open(cache, "w") as f:
+            json.dump(known, f, indent=1, sort_keys=True)
+    return best, {"ms_per_iter": ms.value, "trials": n_trials.value, "cached": False}
+
+
 def em_train(model_type: int, ds: Dataset, iterations: int = 50, device: int = 0, use_graph: bool = True,
              **options):
     """Single-worker em_parallel (parallel_em.cu:349-746): train, then evaluate
