@@ -51,7 +51,8 @@ def cpu_baseline(ds, model, n_sample, iters, threads):
     q = tr.query[idx]
     p = tr.pidx[idx]
     n_qd = int(p.max()) + 1
-    om = oracle.OracleModel(model, q, p, tr.clicks[idx], n_qd, n_threads=threads)
+    ln = None if tr.length is None else tr.length[idx]
+    om = oracle.OracleModel(model, q, p, tr.clicks[idx], n_qd, length=ln, n_threads=threads)
     om.iterate(1)   # warm
     te = tm = 0.0
     for _ in range(iters):
@@ -140,10 +141,15 @@ def calibration_copy(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--config", default="dbn10m", choices=["dbn10m", "batched"],
+                    help="dbn10m: the headline plain-EM workload; batched: 100M sessions of 1-10 ranks, "
+                         "10M Zipf(1.3) queries, autotuned")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--sessions", type=int, default=10_000_000)
-    ap.add_argument("--queries", type=int, default=1_000_000)
+    ap.add_argument("--sessions", type=int, default=None)
+    ap.add_argument("--queries", type=int, default=None)
+    ap.add_argument("--zipf", type=float, default=None)
+    ap.add_argument("--autotune", action="store_true", help="pick the launch options with mc_autotune")
     ap.add_argument("--model", default="dbn", choices=["pbm", "ubm", "dbn", "ccm"])
     ap.add_argument("--mode", default="segmented", choices=["segmented", "atomic"])
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
@@ -157,6 +163,14 @@ def main():
     ap.add_argument("--ttt-lr", type=float, default=0.05)
     ap.add_argument("--ttt-max-epochs", type=int, default=40)
     args = ap.parse_args()
+    batched = args.config == "batched"
+    if batched:
+        args.autotune = True
+        args.ttt_sessions = 0
+    args.steps = args.steps or (20 if batched else 50)
+    args.sessions = args.sessions or (100_000_000 if batched else 10_000_000)
+    args.queries = args.queries or (10_000_000 if batched else 1_000_000)
+    args.zipf = args.zipf or (1.3 if batched else 1.1)
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -171,12 +185,13 @@ def main():
     n_tab = 2 if model == DBN else 1
 
     t0 = time.perf_counter()
-    cfg = synth.SynthConfig(n_sessions=args.sessions, n_queries=args.queries, vocab=20 * args.queries,
-                            zipf_s=1.1, model=args.model, seed=3)
+    cfg = synth.SynthConfig(n_sessions=args.sessions, n_queries=args.queries, vocab=20_000_000,
+                            zipf_s=args.zipf, model=args.model, variable_length=batched, seed=3)
     s = synth.generate(cfg)
-    log(f"generated {args.sessions} sessions in {time.perf_counter() - t0:.1f} s: {s.stats}")
+    stats = s.stats
+    log(f"generated {args.sessions} sessions in {time.perf_counter() - t0:.1f} s: {stats}")
     t0 = time.perf_counter()
-    ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2)
+    ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2, length=s.length if batched else None)
     del s
     log(f"prepared: train {ds.n_train}, test {ds.n_test}, n_qd {ds.n_qd} in {time.perf_counter() - t0:.1f} s")
 
@@ -185,6 +200,14 @@ def main():
         opts["item_sessions"] = args.item_sessions
     if args.block:
         opts["block"] = args.block
+    tune = None
+    if args.autotune:
+        from massiveclicks_amd import autotune
+        t0 = time.perf_counter()
+        opts, tune = autotune(model, ds.train, ds.n_qd, iterations=3, start=opts)
+        tune["options"] = dict(opts)
+        tune["seconds"] = round(time.perf_counter() - t0, 1)
+        log(f"autotune: {tune}")
     cm = ClickModel(model, device=0, **opts)
     t0 = time.perf_counter()
     cm.init_from_dataset(ds)
@@ -202,10 +225,12 @@ def main():
     n_train = ds.n_train
     value = n_train / (ms_iter * 1e-3)
     n_shared = lay["n_shared_entries"]
-    item_bytes = n_train * 42 + 16 * n_tab * (ds.n_qd - (n_shared if args.mode == "segmented" else ds.n_qd))
-    if args.mode == "atomic":
-        item_bytes = n_train * 42
-    iter_bytes = n_train * 42 + 16 * n_tab * ds.n_qd
+    # Session stream: 4 B per shown rank + 2 B click mask (+ 1 B length).
+    mean_len = float(ds.train.length.mean()) if ds.train.length is not None else 10.0
+    sess_bytes = n_train * (4 * mean_len + 2 + (1 if batched else 0))
+    segmented = opts.get("reduce_mode", REDUCE_SEGMENTED) == REDUCE_SEGMENTED
+    item_bytes = sess_bytes + 16 * n_tab * (ds.n_qd - n_shared) if segmented else sess_bytes
+    iter_bytes = sess_bytes + 16 * n_tab * ds.n_qd
     achieved = item_bytes / (kt["ms_estep"] * 1e-3)
 
     ev = cm.evaluate(ds.test)
@@ -213,7 +238,7 @@ def main():
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"{args.model}_{args.sessions // 1_000_000}m_traffic.json")
-    if os.path.exists(tfile) and args.mode == "segmented":
+    if os.path.exists(tfile) and args.mode == "segmented" and not batched and not args.autotune:
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
@@ -224,7 +249,9 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     out = {
-        "metric": METRIC if args.model == "dbn" else f"EM sessions/sec/iter ({args.model.upper()})",
+        "metric": METRIC if (args.model == "dbn" and not batched) else
+        (f"EM sessions/sec/iter (batched uneven {args.model.upper()})" if batched
+         else f"EM sessions/sec/iter ({args.model.upper()})"),
         "value": value,
         "unit": "sessions/s/iter",
         "n_gpus": 1,
@@ -237,13 +264,18 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.model.upper()} plain EM, {args.sessions} synthetic sessions "
-                               f"({n_train} train), {args.queries} Zipf(1.1) queries, pools of 20, "
-                               f"{args.mode} reduction",
+                               f"({n_train} train), {args.queries} Zipf({args.zipf}) queries, pools of 20, "
+                               + ("1-10 ranks per session (uniform), autotuned options" if batched else
+                                  f"{args.mode} reduction" + (", autotuned options" if args.autotune else "")),
                    "sessions": args.sessions, "train_sessions": n_train, "n_qd": ds.n_qd,
-                   "iterations": args.steps, "reduce_mode": args.mode, "work_items": lay["n_items"]},
+                   "mean_ranks": mean_len, "top_query_share": stats.get("top_query_share"),
+                   "iterations": args.steps, "options": {k: v for k, v in opts.items()},
+                   "work_items": lay["n_items"], "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "em_window_persistent_kernel", "bytes_per_launch": item_bytes,
+                     "kernel": ("em_window_persistent_kernel" if segmented and opts.get("persistent", True)
+                                and opts.get("block", 256) == 256 else "em_item_kernel"),
+                     "bytes_per_launch": item_bytes,
                      "kernel_ms": kt["ms_estep"], "merge_kernel_ms": kt["ms_mstep"],
                      "iteration_bytes": iter_bytes,
                      "iteration_frac": iter_bytes / (ms_iter * 1e-3) / HBM_PEAK},
