@@ -46,8 +46,8 @@ enum {
     MC_OPT_ITEM_SESSIONS = 1,
     /* Per-qd accumulator window per work item, in entries (default 1280). */
     MC_OPT_ITEM_WINDOW = 2,
-    /* 1 = reorder sessions inside each work item by last-click rank so that
-     * wavefronts run uniform chains (default 1). */
+    /* 1 = reorder sessions inside each work item by length, then last-click
+     * rank, so that wavefronts run uniform chains (default 1). */
     MC_OPT_SORT_BY_LCR = 3,
     /* Workgroup size of the E/M kernel: 64, 128, 256 or 512 (default 256). */
     MC_OPT_BLOCK = 4,

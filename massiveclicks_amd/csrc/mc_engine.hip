@@ -39,26 +39,51 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 // Kernels
 // ---------------------------------------------------------------------------
 
-template <int MODEL, bool VARLEN, class Src, class Emit>
-__device__ __forceinline__ void run_session(const int (&pv)[kL], int L, uint32_t m, const Src& src, Emit& em,
-                                            const float* sh, float (&acc)[55]) {
-    const uint32_t mm = m & ((1u << L) - 1u);
+template <int MODEL, int LC, class Src, class Emit>
+__device__ __forceinline__ void run_session_len(const int (&pv)[kL], uint32_t m, const Src& src, Emit& em,
+                                                const float* sh, float (&acc)[55]) {
+    constexpr bool VARLEN = LC < kL;
+    const uint32_t mm = m & ((1u << LC) - 1u);
     if (MODEL == kPBM) {
-        pbm_session(pv, L, mm, src, em, sh, acc);
+        pbm_session(pv, LC, mm, src, em, sh, acc);
     } else if (MODEL == kUBM) {
-        ubm_session(pv, L, mm, src, em, sh, acc);
+        ubm_session(pv, LC, mm, src, em, sh, acc);
     } else if (MODEL == kDBN) {
         if (mm == 0) return;   // every count of a click-free session is a constant
         float gn = 0.f, gd = 0.f;
-        dbn_session<VARLEN>(pv, L, mm, src, em, sh[0], gn, gd);
+        dbn_session<VARLEN>(pv, LC, mm, src, em, sh[0], gn, gd);
         acc[0] += gn;
         acc[1] += gd;
     } else {
         if (mm == 0) return;
         float ta[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        ccm_session<VARLEN>(pv, L, mm, src, em, sh[0], sh[1], sh[2], ta);
+        ccm_session<VARLEN>(pv, LC, mm, src, em, sh[0], sh[1], sh[2], ta);
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[k] += ta[k];
+    }
+}
+
+// Uneven lists run the session code specialised for their length, so that
+// every rank loop and tail index is static; sessions of a work item are
+// ordered by length, so a wavefront mostly takes one case.
+template <int MODEL, bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void run_session(const int (&pv)[kL], int L, uint32_t m, const Src& src, Emit& em,
+                                            const float* sh, float (&acc)[55]) {
+    if (!VARLEN) {
+        run_session_len<MODEL, kL>(pv, m, src, em, sh, acc);
+        return;
+    }
+    switch (L) {
+        case 1: run_session_len<MODEL, 1>(pv, m, src, em, sh, acc); break;
+        case 2: run_session_len<MODEL, 2>(pv, m, src, em, sh, acc); break;
+        case 3: run_session_len<MODEL, 3>(pv, m, src, em, sh, acc); break;
+        case 4: run_session_len<MODEL, 4>(pv, m, src, em, sh, acc); break;
+        case 5: run_session_len<MODEL, 5>(pv, m, src, em, sh, acc); break;
+        case 6: run_session_len<MODEL, 6>(pv, m, src, em, sh, acc); break;
+        case 7: run_session_len<MODEL, 7>(pv, m, src, em, sh, acc); break;
+        case 8: run_session_len<MODEL, 8>(pv, m, src, em, sh, acc); break;
+        case 9: run_session_len<MODEL, 9>(pv, m, src, em, sh, acc); break;
+        default: run_session_len<MODEL, 10>(pv, m, src, em, sh, acc); break;
     }
 }
 
@@ -69,6 +94,28 @@ template <int MODEL, bool VARLEN, class Src, class Emit>
 __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const Src& src, Emit& em, const float* sh,
                                          float (&acc)[55], uint32_t m, int L, int (&pv)[kL]) {
     const int nthr = blockDim.x;
+    if (VARLEN) {
+        // Two-stage prefetch: the length of session s + 2 nthr arrives one
+        // session ahead of its indices, which are read only up to it.
+        uint32_t mn;
+        int Ln;
+        load_head(st, it.s0 + (int)threadIdx.x + nthr, it.s1, mn, Ln);
+        for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
+            uint32_t m3;
+            int L3;
+            int pv2[kL];
+            load_head(st, s + 2 * nthr, it.s1, m3, L3);
+            load_pidx_upto(st, s + nthr, it.s1, Ln, pv2);
+            run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
+            m = mn;
+            L = Ln;
+            mn = m3;
+            Ln = L3;
+#pragma unroll
+            for (int r = 0; r < kL; r++) pv[r] = pv2[r];
+        }
+        return;
+    }
     for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
         uint32_t m2;
         int L2;
@@ -1173,7 +1220,9 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
         for (const auto& rg : sp.ranges)
             for (int64_t s = rg.first; s < rg.second; s++) order.push_back(s);
         if (m->sort_by_lcr)
-            std::stable_sort(order.begin() + j0, order.end(), [&](int64_t a, int64_t b) { return lcr[a] < lcr[b]; });
+            std::stable_sort(order.begin() + j0, order.end(), [&](int64_t a, int64_t b) {
+                return L[a] != L[b] ? L[a] < L[b] : lcr[a] < lcr[b];
+            });
         items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, batch_of(sp.gstart)});
         if (sp.kind == kKindWindow && m->reduce_mode == MC_REDUCE_SEGMENTED) m->n_win_items++;
         else if (sp.kind == kKindDirect) m->n_dir_items++;

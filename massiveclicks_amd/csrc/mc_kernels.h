@@ -504,6 +504,25 @@ __device__ __forceinline__ void load_session(const Stream& st, int s, int s_end,
     for (int r = 0; r < kL; r++) pv[r] = (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, o * 4u, r * st.row_bytes, 0);
 }
 
+// Click bits and length of one session (index clamped as in load_session).
+__device__ __forceinline__ void load_head(const Stream& st, int s, int s_end, uint32_t& m, int& L) {
+    const uint32_t o = (uint32_t)min(s, s_end - 1);
+    m = __builtin_amdgcn_raw_buffer_load_b16(st.mask, o * 2u, 0, 0);
+    L = (int)__builtin_amdgcn_raw_buffer_load_b8(st.len, o, 0, 0);
+}
+
+// Query-document indices of the first L ranks of one session; ranks past L
+// are not read (a wavefront whose sessions all end before rank r issues no
+// load for it).
+__device__ __forceinline__ void load_pidx_upto(const Stream& st, int s, int s_end, int L, int (&pv)[kL]) {
+    const uint32_t o = (uint32_t)min(s, s_end - 1);
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        pv[r] = 0;
+        if (r < L) pv[r] = (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, o * 4u, r * st.row_bytes, 0);
+    }
+}
+
 // Wave-level sum of a 64-bit integer (wave64).
 __device__ __forceinline__ long long wave_sum(long long v) {
 #pragma unroll
