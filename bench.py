@@ -150,6 +150,9 @@ def main():
     ap.add_argument("--queries", type=int, default=None)
     ap.add_argument("--zipf", type=float, default=None)
     ap.add_argument("--autotune", action="store_true", help="pick the launch options with mc_autotune")
+    ap.add_argument("--no-autotune", action="store_true", help="batched configuration with the given options")
+    ap.add_argument("--persistent", type=int, default=None, choices=[0, 1])
+    ap.add_argument("--item-window", type=int, default=None)
     ap.add_argument("--model", default="dbn", choices=["pbm", "ubm", "dbn", "ccm"])
     ap.add_argument("--mode", default="segmented", choices=["segmented", "atomic"])
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
@@ -165,7 +168,7 @@ def main():
     args = ap.parse_args()
     batched = args.config == "batched"
     if batched:
-        args.autotune = True
+        args.autotune = not args.no_autotune
         args.ttt_sessions = 0
     args.steps = args.steps or (20 if batched else 50)
     args.sessions = args.sessions or (100_000_000 if batched else 10_000_000)
@@ -200,6 +203,10 @@ def main():
         opts["item_sessions"] = args.item_sessions
     if args.block:
         opts["block"] = args.block
+    if args.persistent is not None:
+        opts["persistent"] = bool(args.persistent)
+    if args.item_window:
+        opts["item_window"] = args.item_window
     tune = None
     if args.autotune:
         from massiveclicks_amd import autotune
@@ -265,7 +272,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.model.upper()} plain EM, {args.sessions} synthetic sessions "
                                f"({n_train} train), {args.queries} Zipf({args.zipf}) queries, pools of 20, "
-                               + ("1-10 ranks per session (uniform), autotuned options" if batched else
+                               + ("1-10 ranks per session (uniform)" + (", autotuned options" if args.autotune else "")
+                                  if batched else
                                   f"{args.mode} reduction" + (", autotuned options" if args.autotune else "")),
                    "sessions": args.sessions, "train_sessions": n_train, "n_qd": ds.n_qd,
                    "mean_ranks": mean_len, "top_query_share": stats.get("top_query_share"),
