@@ -169,12 +169,14 @@ __device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st
 // Session paths, one kernel instantiation each (each gets its own register
 // allocation): window items in LDS, items without a window (global 64-bit
 // fixed-point atomics), and the float-atomic mode.
-enum { kPathLds = 0, kPathGlobal = 1, kPathAtomic = 2, kPathDirect = 3 };
+enum { kPathLds = 0, kPathGlobal = 1, kPathAtomic = 2, kPathDirect = 3, kPathLdsFloat = 4 };
 
 // E-step and in-item M-step of one work item (item index item_offset +
 // blockIdx.x).  Dynamic LDS (window path): width x T fixed-point
 // accumulators, then width x T staged values of theta_t (both interleaved per
-// entry).
+// entry).  kPathLdsFloat (float-atomic mode) pre-reduces in float LDS
+// accumulators and adds every window entry to global float accumulators,
+// which the merge kernel finalises.
 template <int MODEL, bool VARLEN, int PATH>
 __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) {
     constexpr int T = Shape<MODEL>::T;
@@ -188,9 +190,11 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
     const int W = it.width;
-    constexpr bool win = PATH == kPathLds;
+    constexpr bool fwin = PATH == kPathLdsFloat;
+    constexpr bool win = PATH == kPathLds || fwin;
     unsigned long long* lacc = lds;
-    float* lval = (float*)(lds + T * W);
+    float* facc = (float*)lds;
+    float* lval = fwin ? facc + T * W : (float*)(lds + T * W);
 
     // First session's loads go out before the window is staged.
     const Stream st = make_stream(A);
@@ -203,7 +207,8 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
         for (int i = tid; i < W; i += nthr) {
 #pragma unroll
             for (int t = 0; t < T; t++) {
-                lacc[i * T + t] = 0ull;
+                if (fwin) facc[i * T + t] = 0.f;
+                else lacc[i * T + t] = 0ull;
                 lval[i * T + t] = A.tab[t].val_in[it.base + i];
             }
         }
@@ -220,6 +225,10 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
         run_item<MODEL, VARLEN>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLds) {
         EmitLds<T> em{lacc};
+        const SrcLds<T> lsrc{lval};
+        run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+    } else if (PATH == kPathLdsFloat) {
+        EmitLdsFloat<T> em{facc};
         const SrcLds<T> lsrc{lval};
         run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathDirect) {
@@ -244,9 +253,18 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
         if (v != 0) atomicAdd(&A.sh_acc[k], (unsigned long long)v);
     }
 
+    if (fwin) {
+        for (int i = tid; i < W; i += nthr) {
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                const float v = facc[i * T + t];
+                if (v != 0.f) unsafeAtomicAdd(&A.tab[t].gaccf[it.base + i], v);
+            }
+        }
+    }
     // Flush the LDS window, four entries per thread per pass with the
     // constant loads issued together.
-    if (win) {
+    if (win && !fwin) {
         const bool interior = (it.flags & kItemInterior) != 0;
         for (int i0 = 0; i0 < W; i0 += 4 * nthr) {
             int2 oc[4];
@@ -721,7 +739,10 @@ constexpr int kPersistPer = 5;
 template <int MODEL>
 void launch_item(mc_model* m, const KArgs& A) {
     if (m->reduce_mode == MC_REDUCE_ATOMIC) {
-        launch_path<MODEL, kPathAtomic>(m, A, 0, m->n_items, 0);
+        // LDS pre-reduction of window items, then float atomics; items
+        // without a window add straight to the global accumulators.
+        launch_path<MODEL, kPathLdsFloat>(m, A, 0, m->n_win_items, m->lds_bytes);
+        launch_path<MODEL, kPathAtomic>(m, A, m->n_win_items, m->n_glob_items, 0);
     } else {
         if (m->persistent && m->n_win_items > 0) {
             if (m->varlen)
@@ -1096,7 +1117,10 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     // groups, cut by session count.
     const int B = m->n_batches;
     auto batch_of = [&](int64_t first_session) { return (int)std::min<int64_t>(B - 1, first_session * B / n); };
-    bool structured = m->reduce_mode == MC_REDUCE_SEGMENTED;
+    // Both reduction modes use query-group work items when the training set
+    // is grouped by query; the float-atomic mode merges every entry.
+    bool structured = true;
+    const bool allow_direct = m->direct && m->reduce_mode == MC_REDUCE_SEGMENTED;
     if (structured) {
         // Query groups: runs of equal query ids.  The segmented reduction needs
         // each group's entries in a range [lo, hi) disjoint from and after the
@@ -1140,7 +1164,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
             for (const Group& g : groups) {
                 const int64_t gs = g.s1 - g.s0;
                 const int64_t gw = g.hi - g.lo;
-                if (m->direct && gs == 1 && distinct(g.s0)) {
+                if (allow_direct && gs == 1 && distinct(g.s0)) {
                     if (dir.count == 0) dir.gstart = g.s0;
                     dir.ranges.push_back({g.s0, g.s1});
                     if (++dir.count == S_dir) close_dir();
@@ -1224,7 +1248,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
                 return L[a] != L[b] ? L[a] < L[b] : lcr[a] < lcr[b];
             });
         items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, batch_of(sp.gstart)});
-        if (sp.kind == kKindWindow && m->reduce_mode == MC_REDUCE_SEGMENTED) m->n_win_items++;
+        if (sp.kind == kKindWindow && structured) m->n_win_items++;
         else if (sp.kind == kKindDirect) m->n_dir_items++;
         else m->n_glob_items++;
     }
@@ -1293,7 +1317,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     int max_width = 0;
     for (const Item& it : items) max_width = std::max(max_width, (int)it.width);
     m->max_width = max_width;
-    m->lds_bytes = m->reduce_mode == MC_REDUCE_SEGMENTED ? (size_t)max_width * T * 12 : 0;
+    m->lds_bytes = (size_t)max_width * T * (m->reduce_mode == MC_REDUCE_SEGMENTED ? 12 : 8);
 
     // Persistent window kernel: 256-thread workgroups, windows staged through
     // registers (width <= 256 * kPersistPer), one workgroup per resident slot.
@@ -1324,7 +1348,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     // base (the kernel addresses the LDS window with them); other sessions
     // store absolute indices.
     for (const Item& it : items) {
-        const int32_t off = (m->reduce_mode == MC_REDUCE_SEGMENTED && it.width > 0) ? it.base : 0;
+        const int32_t off = (structured && it.width > 0) ? it.base : 0;
         for (int64_t j = it.s0; j < it.s1; j++) {
             const int64_t s = order[j];
             for (int r = 0; r < kMaxSerp; r++)

@@ -141,6 +141,11 @@ struct EmitGlobalFix {      // fixed-point global (item without a window)
         atomicAdd(t == 0 ? &g0[p] : &g1[p], (unsigned long long)to_fix(x));
     }
 };
+template <int T>
+struct EmitLdsFloat {       // float LDS window (float-atomic mode), relative indices
+    float* acc;                // [width][T]
+    __device__ __forceinline__ void add(int t, int, int p, float x) { atomicAdd(&acc[p * T + t], x); }
+};
 struct EmitGlobalFloat {    // float atomics (reference-like mode)
     float* g0;
     float* g1;
