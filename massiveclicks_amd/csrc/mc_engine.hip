@@ -578,24 +578,24 @@ template <int MODEL>
 __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
     __shared__ double red[21][4];
     const int tid = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
     double v[21];
 #pragma unroll
     for (int k = 0; k < 21; k++) v[k] = 0.0;
-    if (i < E.n) {
+    // Grid-stride: each lane sums several sessions before the block reduction.
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < E.n; i += (int64_t)gridDim.x * blockDim.x) {
         const int L = E.len ? (int)E.len[i] : kL;
         float lp[kL], fp[kL];
         eval_session<MODEL>(E, i, L, lp, fp);
         float acc = 0.f;
         for (int r = 0; r < L; r++) acc = acc + lp[r];
         const float s = acc / L;                 // evaluation.cpp:36
-        v[0] = (double)s;
+        v[0] += (double)s;
         if (E.llh) E.llh[i] = s;
 #pragma unroll
         for (int r = 0; r < kL; r++) {
             if (r < L) {
-                v[1 + r] = (double)log2f(fp[r]);   // evaluation.cpp:62
-                v[11 + r] = 1.0;
+                v[1 + r] += (double)log2f(fp[r]);   // evaluation.cpp:62
+                v[11 + r] += 1.0;
             }
             if (E.logp) E.logp[i * kL + r] = r < L ? lp[r] : 0.f;
             if (E.full) E.full[i * kL + r] = r < L ? fp[r] : 0.f;
@@ -1646,7 +1646,7 @@ mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t
         }
     }
     // Temporary device buffers for this call.
-    const int64_t nb = (n + 255) / 256;
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
     std::vector<void*> tmp;
     auto talloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
