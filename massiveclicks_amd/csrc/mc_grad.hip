@@ -36,6 +36,7 @@ namespace mc {
 namespace {
 
 constexpr float kLogitMax = 13.8f;   // sigmoid(13.8) = 1 - 1.0e-6, the Param::value clamp
+constexpr int kShAccStride = 64;     // shared-table sums of one step (>= 55 accumulators)
 
 struct GArgs {
     const int32_t* pidx;
@@ -54,17 +55,28 @@ struct GArgs {
     float* sh_val;
     float* sh_num;
     float* sh_den;
-    float* sh_logit;
+    float* sh_logit;               // Adam state of the shared tables: the version read
     float* sh_am;
     float* sh_av;
+    float* wr_logit;               // ... and the version written (may be the same)
+    float* wr_am;
+    float* wr_av;
+    float* wr_val;
+    int32_t wr_main;               // the written version is the model's own (also set num/den)
     float* sh_grad;
     const double* sh_prior;
     unsigned long long* sh_acc;
     double* ll_part;               // per item
     const int32_t* shared_idx;
-    int32_t* steps;                // Adam steps per batch, then of the shared tables
-    unsigned* merge_done;          // blocks of the current merge that have finished
+    int32_t* steps;                // [0]: completed epochs (every batch once)
     int64_t shared_off, shared_cnt;  // split entries of the batch
+    // Fused finalisation of the previous step (Adam epochs replayed from a
+    // graph): its batch (-1: none), split entries, shared-table sums, scale.
+    int32_t prev_batch;
+    int64_t prev_off, prev_cnt;
+    double prev_scale;
+    const unsigned long long* prev_acc;
+    unsigned long long* zero_acc;  // shared sums cleared by workgroup 0 (read two steps ago)
     int32_t K, kacc, max_width, batch, n_batches, use_prior;
     float lr, b1, b2, eps;
     double sh_scale;               // training sessions / batch sessions
@@ -263,27 +275,40 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // Merge work of grad_merge_kernel.
 __device__ __forceinline__ unsigned long long load_sum(const unsigned long long* p) { return *p; }
 
-template <int MODEL, bool ADAM>
-__device__ __forceinline__ void merge_shared_entry(const GArgs& G, int k) {
-    const double sum = fix_to_double(load_sum(&G.sh_acc[k]));
-    if (ADAM) {
-        const float val = G.sh_val[k];
-        double g = sum * G.sh_scale;
-        if (G.use_prior) g += G.sh_prior[k] * (1.0 - 2.0 * (double)val);
-        const AdamStep ad = adam_step(G, G.steps[G.n_batches] + 1);
-        const float nv = ad.apply(G.sh_logit[k], G.sh_am[k], G.sh_av[k], (float)g);
-        G.sh_val[k] = nv;
-        G.sh_num[k] = nv;
-        G.sh_den[k] = 1.f;
-    } else {
-        G.sh_grad[k] = (float)sum;
+// Adam step of shared-table entry k from the sum of a step of batch
+// `batch` (scaled by `scale`); returns the new value and, when `write`,
+// stores the new state in the written version.
+__device__ __forceinline__ float shared_adam(const GArgs& G, int k, double sum, double scale, int batch, bool write) {
+    const float val = G.sh_val[k];
+    double g = sum * scale;
+    if (G.use_prior) g += G.sh_prior[k] * (1.0 - 2.0 * (double)val);
+    const AdamStep ad = adam_step(G, G.steps[0] * G.n_batches + batch + 1);
+    float logit = G.sh_logit[k], am = G.sh_am[k], av = G.sh_av[k];
+    const float nv = ad.apply(logit, am, av, (float)g);
+    if (write) {
+        G.wr_logit[k] = logit;
+        G.wr_am[k] = am;
+        G.wr_av[k] = av;
+        G.wr_val[k] = nv;
+        if (G.wr_main) {
+            G.sh_num[k] = nv;
+            G.sh_den[k] = 1.f;
+        }
     }
+    return nv;
 }
 
 template <int MODEL, bool ADAM>
-__device__ __forceinline__ void merge_split_entry(const GArgs& G, int64_t i, const AdamStep& ad) {
+__device__ __forceinline__ void merge_shared_entry(const GArgs& G, int k) {
+    const double sum = fix_to_double(load_sum(&G.sh_acc[k]));
+    if (ADAM) shared_adam(G, k, sum, G.sh_scale, G.batch, true);
+    else G.sh_grad[k] = (float)sum;
+}
+
+template <int MODEL, bool ADAM>
+__device__ __forceinline__ void merge_split_entry(const GArgs& G, int64_t off, int64_t i, const AdamStep& ad) {
     constexpr int T = Shape<MODEL>::T;
-    const int p = G.shared_idx[G.shared_off + i];
+    const int p = G.shared_idx[off + i];
     const int2 oc = G.counts[p];
 #pragma unroll
     for (int t = 0; t < T; t++) {
@@ -293,28 +318,6 @@ __device__ __forceinline__ void merge_split_entry(const GArgs& G, int64_t i, con
         else G.grad[t][p] = g;
         G.gacc[t][p] = 0ull;
     }
-}
-
-// The last workgroup of a launch to arrive (counted in merge_done) returns
-// true, after every other workgroup has finished its reads and atomics.
-__device__ __forceinline__ bool last_block(const GArgs& G) {
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        is_last = atomicAdd(G.merge_done, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    return is_last != 0;
-}
-
-// Step counts and the arrival count, by one thread of the last workgroup.
-__device__ __forceinline__ void finish_step(const GArgs& G, bool adam) {
-    if (adam) {
-        atomicAdd(&G.steps[G.batch], 1);
-        atomicAdd(&G.steps[G.n_batches], 1);
-    }
-    atomicExch(G.merge_done, 0u);
 }
 
 // One workgroup per listed item: the sessions' gradients, reduced in the
@@ -349,6 +352,18 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
     int pv0[kL];
     load_session<VARLEN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
     for (int k = tid; k < K; k += NT) sh[k] = G.sh_val[k];
+    if (ADAM && G.prev_batch >= 0) {
+        // The previous step's finalisation: every workgroup computes the
+        // shared tables' new values for its own sessions (workgroup 0 stores
+        // them), and the workgroups share the previous batch's split entries.
+        for (int k = tid; k < K; k += NT)
+            sh[k] = shared_adam(G, k, fix_to_double(G.prev_acc[k]), G.prev_scale, G.prev_batch, blockIdx.x == 0);
+        if (blockIdx.x == 0)
+            for (int j = tid; j < G.kacc; j += NT) G.zero_acc[j] = 0ull;
+        const AdamStep pad = adam_step(G, G.steps[0] + 1);
+        for (int64_t i = (int64_t)blockIdx.x * NT + tid; i < G.prev_cnt; i += (int64_t)gridDim.x * NT)
+            merge_split_entry<MODEL, true>(G, G.prev_off, i, pad);
+    }
     if (WIN) {
         for (int i = tid; i < W; i += NT) {
 #pragma unroll
@@ -359,6 +374,27 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
         }
     }
     __syncthreads();
+
+    // Wide workgroups (small minibatches, one round of items) fetch the
+    // inputs of their entries' finalisation now, so that they arrive during
+    // the session loop: lane tid owns entries tid + u * NT, u < PER.
+    constexpr int PER = NT >= 1024 ? 2 : 0;
+    constexpr int PERA = PER > 0 ? PER : 1;
+    const bool interior = WIN && (it.flags & kItemInterior) != 0;
+    int2 f_oc[PERA];
+    float f_logit[PERA][T], f_m[PERA][T], f_v[PERA][T];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int i = u * NT + tid;
+        const bool on = interior && i < W;
+        f_oc[u] = on ? G.counts[it.base + i] : make_int2(0, 0);
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            f_logit[u][t] = (ADAM && on) ? G.logit[t][it.base + i] : 0.f;
+            f_m[u][t] = (ADAM && on) ? G.am[t][it.base + i] : 0.f;
+            f_v[u][t] = (ADAM && on) ? G.av[t][it.base + i] : 0.f;
+        }
+    }
 
     float acc[55];
 #pragma unroll
@@ -410,10 +446,28 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
     }
 
     if (WIN) {
-        const bool interior = (it.flags & kItemInterior) != 0;
         AdamStep ad;
-        if (ADAM && interior) ad = adam_step(G, G.steps[G.batch] + 1);
-        for (int i = tid; i < W; i += NT) {
+        if (ADAM && interior) ad = adam_step(G, G.steps[0] + 1);
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int i = u * NT + tid;
+            if (!interior || i >= W) continue;
+            const int p = it.base + i;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                const float g = entry_gradient<MODEL>(t, fix_to_double(lacc[i * T + t]), f_oc[u], lval[i * T + t],
+                                                      ADAM && G.use_prior);
+                if (ADAM) {
+                    G.val[t][p] = ad.apply(f_logit[u][t], f_m[u][t], f_v[u][t], g);
+                    G.logit[t][p] = f_logit[u][t];
+                    G.am[t][p] = f_m[u][t];
+                    G.av[t][p] = f_v[u][t];
+                } else {
+                    G.grad[t][p] = g;
+                }
+            }
+        }
+        for (int i = tid + (interior ? PER * NT : 0); i < W; i += NT) {
             const int p = it.base + i;
             const int2 oc = interior ? G.counts[p] : make_int2(0, 0);
 #pragma unroll
@@ -444,13 +498,21 @@ __global__ void __launch_bounds__(256) grad_merge_kernel(GArgs G) {
         if (tid < G.K) merge_shared_entry<MODEL, ADAM>(G, tid);
         __syncthreads();
         for (int j = tid; j < G.kacc; j += blockDim.x) G.sh_acc[j] = 0ull;
+        if (G.zero_acc)
+            for (int j = tid; j < 3 * kShAccStride; j += blockDim.x) G.zero_acc[j] = 0ull;
     } else {
         const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
         AdamStep ad;
-        if (ADAM) ad = adam_step(G, G.steps[G.batch] + 1);
-        if (i < G.shared_cnt) merge_split_entry<MODEL, ADAM>(G, i, ad);
+        if (ADAM) ad = adam_step(G, G.steps[0] + 1);
+        if (i < G.shared_cnt) merge_split_entry<MODEL, ADAM>(G, G.shared_off, i, ad);
     }
-    if (last_block(G) && tid == 0) finish_step(G, ADAM);
+}
+
+// Counts a completed epoch; launched after the step of the last batch.  A
+// step of batch b in epoch e is Adam step e + 1 of b's entries and step
+// e * batches + b + 1 of the shared tables.
+__global__ void advance_epoch_kernel(int32_t* steps) {
+    if (threadIdx.x == 0) steps[0] += 1;
 }
 
 // logit = log(v / (1 - v)) of every entry; moments to zero.
@@ -499,11 +561,14 @@ mc_status ensure_grad_buffers(mc_model* m) {
     MC_TRY(dalloc(m, &m->d_sh_logit, (size_t)K * 4));
     MC_TRY(dalloc(m, &m->d_sh_am, (size_t)K * 4));
     MC_TRY(dalloc(m, &m->d_sh_av, (size_t)K * 4));
+    MC_TRY(dalloc(m, &m->d_sh2, (size_t)4 * kShAccStride * 4));     // alternate logit, m, v, value
+    MC_TRY(dalloc(m, &m->d_gsh, (size_t)3 * kShAccStride * 8));     // three steps of shared-table sums
+    MC_HIP(hipMemset(m->d_gsh, 0, (size_t)3 * kShAccStride * 8));
     MC_TRY(dalloc(m, &m->d_sh_grad, (size_t)K * 4));
     MC_TRY(dalloc(m, &m->d_sh_prior, (size_t)K * 8));
     MC_TRY(dalloc(m, &m->d_ll_part, (size_t)m->n_items * 8));
-    MC_TRY(dalloc(m, &m->d_steps, (size_t)(m->n_batches + 2) * 4));   // counts, then the merge's finished-block count
-    MC_HIP(hipMemset(m->d_steps, 0, (size_t)(m->n_batches + 2) * 4));
+    MC_TRY(dalloc(m, &m->d_steps, 4));
+    MC_HIP(hipMemset(m->d_steps, 0, 4));
     // Prior weights of the shared tables: 1 + the sessions that leave the
     // slot's temporary at (1,2) (the phantom count in sh_base).
     std::vector<double> base(2 * (size_t)K), prior((size_t)K);
@@ -547,13 +612,18 @@ GArgs make_gargs(mc_model* m, int batch) {
     G.sh_logit = m->d_sh_logit;
     G.sh_am = m->d_sh_am;
     G.sh_av = m->d_sh_av;
+    G.wr_logit = m->d_sh_logit;
+    G.wr_am = m->d_sh_am;
+    G.wr_av = m->d_sh_av;
+    G.wr_val = m->d_sh_val[m->cur];
+    G.wr_main = 1;
+    G.prev_batch = -1;
     G.sh_grad = m->d_sh_grad;
     G.sh_prior = m->d_sh_prior;
-    G.sh_acc = m->d_sh_acc;
+    G.sh_acc = m->d_gsh;
     G.ll_part = m->d_ll_part;
     G.shared_idx = m->d_shared_idx;
     G.steps = m->d_steps;
-    G.merge_done = (unsigned*)(m->d_steps + m->n_batches + 1);
     G.K = m->K;
     G.kacc = m->KACC;
     G.max_width = m->max_width;
@@ -599,6 +669,91 @@ void launch_grad_t(mc_model* m, const GArgs& G0, int batch) {
     }
     const unsigned grid = (unsigned)(1 + (Gb.shared_cnt + 255) / 256);
     hipLaunchKernelGGL((grad_merge_kernel<MODEL, ADAM>), dim3(grid), dim3(256), 0, m->stream, Gb);
+    if (ADAM && batch == m->n_batches - 1) hipLaunchKernelGGL(advance_epoch_kernel, dim3(1), dim3(64), 0, m->stream, G.steps);
+}
+
+// One Adam step of an epoch replayed from a graph: the step of batch b also
+// finalises the step of batch b - 1 (its split entries and the shared
+// tables), so an epoch is one launch per step plus a closing merge.  The
+// shared tables' Adam state alternates between the model's arrays (version
+// A) and d_sh2 (version B): step j >= 1 writes version j % 2 == 1 ? B : A,
+// and the closing merge writes A.  Shared sums of step j go to d_gsh slot
+// j % 3; step j clears slot (j + 1) % 3, last read by step j - 1.
+template <int MODEL>
+void launch_fused_step_t(mc_model* m, int b) {
+    const int B = m->n_batches;
+    float* A4[4] = {m->d_sh_logit, m->d_sh_am, m->d_sh_av, m->d_sh_val[m->cur]};
+    float* B4[4] = {m->d_sh2, m->d_sh2 + kShAccStride, m->d_sh2 + 2 * kShAccStride, m->d_sh2 + 3 * kShAccStride};
+    auto version = [&](int j) { return (j % 2 == 1) ? B4 : A4; };
+    GArgs G = make_gargs(m, b);
+    G.shared_off = m->shared_off[(size_t)b];
+    G.shared_cnt = m->shared_cnt[(size_t)b];
+    if (b < B) {
+        float** rd = version(b > 0 ? b - 1 : 0);
+        float** wr = version(b);
+        G.sh_logit = rd[0];
+        G.sh_am = rd[1];
+        G.sh_av = rd[2];
+        G.sh_val = rd[3];
+        G.wr_logit = wr[0];
+        G.wr_am = wr[1];
+        G.wr_av = wr[2];
+        G.wr_val = wr[3];
+        G.wr_main = wr == A4;
+        G.sh_acc = m->d_gsh + (size_t)(b % 3) * kShAccStride;
+        if (b > 0) {
+            G.prev_batch = b - 1;
+            G.prev_off = m->shared_off[(size_t)b - 1];
+            G.prev_cnt = m->shared_cnt[(size_t)b - 1];
+            const int64_t nb = m->batch_sessions[(size_t)b - 1];
+            G.prev_scale = nb > 0 ? (double)m->N / (double)nb : 1.0;
+            G.prev_acc = m->d_gsh + (size_t)((b - 1) % 3) * kShAccStride;
+            G.zero_acc = m->d_gsh + (size_t)((b + 1) % 3) * kShAccStride;
+        }
+        const size_t bi = (size_t)b;
+        const int64_t nw = m->win_cnt[bi], ng = m->glob_cnt[bi];
+        const bool wide = nw + ng < 4 * (int64_t)m->n_cu;
+        if (nw > 0) {
+            if (wide) launch_items<MODEL, true, true, 1024>(m, G, nw, m->win_off[bi], m->lds_bytes);
+            else launch_items<MODEL, true, true, 256>(m, G, nw, m->win_off[bi], m->lds_bytes);
+        }
+        if (ng > 0) {
+            // A second launch of the same step: the finalisation is done and
+            // the shared values are in the written version.
+            if (nw > 0) {
+                G.prev_batch = -1;
+                G.sh_val = G.wr_val;
+            }
+            if (wide) launch_items<MODEL, false, true, 1024>(m, G, ng, m->glob_off[bi], 0);
+            else launch_items<MODEL, false, true, 256>(m, G, ng, m->glob_off[bi], 0);
+        }
+        return;
+    }
+    // b == B: the closing merge of batch B - 1, into version A.
+    float** rd = version(B - 1);
+    G = make_gargs(m, B - 1);
+    G.shared_off = m->shared_off[(size_t)B - 1];
+    G.shared_cnt = m->shared_cnt[(size_t)B - 1];
+    G.sh_logit = rd[0];
+    G.sh_am = rd[1];
+    G.sh_av = rd[2];
+    G.sh_val = rd[3];
+    G.sh_acc = m->d_gsh + (size_t)((B - 1) % 3) * kShAccStride;
+    G.zero_acc = m->d_gsh;
+    const unsigned grid = (unsigned)(1 + (G.shared_cnt + 255) / 256);
+    hipLaunchKernelGGL((grad_merge_kernel<MODEL, true>), dim3(grid), dim3(256), 0, m->stream, G);
+    hipLaunchKernelGGL(advance_epoch_kernel, dim3(1), dim3(64), 0, m->stream, G.steps);
+}
+
+mc_status launch_fused_step(mc_model* m, int b) {
+    switch (m->type) {
+        case MC_PBM: launch_fused_step_t<kPBM>(m, b); break;
+        case MC_UBM: launch_fused_step_t<kUBM>(m, b); break;
+        case MC_DBN: launch_fused_step_t<kDBN>(m, b); break;
+        default: launch_fused_step_t<kCCM>(m, b); break;
+    }
+    MC_HIP(hipGetLastError());
+    return MC_OK;
 }
 
 mc_status launch_grad(mc_model* m, int batch, bool adam) {
@@ -633,8 +788,7 @@ mc_status mc_grad_init(mc_model* m) {
                            m->d_val[m->cur][t], m->d_logit[t], m->d_am[t], m->d_av[t], m->n_qd);
     hipLaunchKernelGGL(logit_init_kernel, dim3(1), dim3(256), 0, m->stream, m->d_sh_val[m->cur], m->d_sh_logit,
                        m->d_sh_am, m->d_sh_av, (int64_t)m->K);
-    hipLaunchKernelGGL(zero_steps_kernel, dim3(1 + m->n_batches / 256), dim3(256), 0, m->stream, m->d_steps,
-                       m->n_batches + 1);
+    hipLaunchKernelGGL(zero_steps_kernel, dim3(1), dim3(64), 0, m->stream, m->d_steps, 1);
     MC_HIP(hipGetLastError());
     MC_HIP(hipStreamSynchronize(m->stream));
     m->adam_next_batch = 0;
@@ -698,7 +852,7 @@ mc_status mc_adam_steps(mc_model* m, int n_steps, int use_graph) {
             hipGraph_t g;
             MC_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
             mc_status st = MC_OK;
-            for (int b = 0; b < B && st == MC_OK; b++) st = launch_grad(m, b, true);
+            for (int b = 0; b <= B && st == MC_OK; b++) st = launch_fused_step(m, b);
             hipError_t e = hipStreamEndCapture(m->stream, &g);
             if (st != MC_OK) return st;
             MC_HIP(e);

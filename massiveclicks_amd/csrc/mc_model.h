@@ -101,9 +101,11 @@ struct mc_model {
     float* d_sh_am = nullptr;
     float* d_sh_av = nullptr;
     float* d_sh_grad = nullptr;
+    float* d_sh2 = nullptr;                    // alternate Adam state of the shared tables (fused epochs)
+    unsigned long long* d_gsh = nullptr;       // shared-table sums of the gradient path, three steps
     double* d_sh_prior = nullptr;              // per shared entry: pseudo-count weight of the prior
     double* d_ll_part = nullptr;               // per item: log-likelihood of its sessions
-    int32_t* d_steps = nullptr;                // Adam step count per batch, then of the shared tables
+    int32_t* d_steps = nullptr;                // completed Adam epochs (every batch once)
     int n_cu = 0;
     float adam_lr = 0.05f, adam_b1 = 0.9f, adam_b2 = 0.999f, adam_eps = 1e-8f;
     int adam_prior = 1;
