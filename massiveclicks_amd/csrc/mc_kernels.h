@@ -384,35 +384,40 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
                                             float t1, float t2, float t3, float (&ta)[6]) {
 #pragma clang fp contract(fast)
     const int lcr = last_click(mm);
-    float av[kL];
+    // Pass 1, ranks 0..L-1: exam[r] (compute_exm_car) and the tail products
+    // V_k of the fresh-list chain (G_{k+1} = G_k tau1 (1 - a_k)).
+    float exam[kL], V[kL];
+    {
+        float e = 1.f, v = 1.f, G = 1.f;
 #pragma unroll
-    for (int r = 0; r < kL; r++) {
-        if (!VARLEN || r < L) {
-            av[r] = src.get(0, pv[r]);
-        } else {
-            av[r] = 0.f;
+        for (int r = 0; r < kL; r++) {
+            exam[r] = e;
+            V[r] = v;
+            if (!VARLEN || r < L) {
+                const float a = src.get(0, pv[r]);
+                const float t = (1 - a) * t1;
+                e = e * (t + a * ((1 - a) * t2 + a * t3));
+                v = v - a * G;
+                G = G * t;
+            }
         }
     }
-    float t[kL], exam[kL + 1], car[kL + 1];
-    exam[0] = 1.f;
+    // Pass 2, ranks L-1 down to lcr+1 (all unclicked): car[r] and the
+    // attractiveness posterior (1-exam)a/(1-exam car); car[lcr+1] is kept
+    // for the last click.  Parameters are read again from LDS.
+    __asm__ volatile("" ::: "memory");
+    float car = 0.f;
 #pragma unroll
-    for (int r = 0; r < kL; r++) {
-        t[r] = (1 - av[r]) * t1;
-        exam[r + 1] = exam[r] * (t[r] + av[r] * ((1 - av[r]) * t2 + av[r] * t3));
+    for (int r = kL - 1; r >= 0; r--) {
+        if ((!VARLEN || r < L) && r > lcr) {
+            const float a = src.get(0, pv[r]);
+            car = a + (1 - a) * t1 * car;
+            em.add(0, r, pv[r], fdiv((1 - exam[r]) * a, 1 - exam[r] * car));
+        }
     }
-    car[kL] = 0.f;
-#pragma unroll
-    for (int r = kL - 1; r >= 0; r--) car[r] = (!VARLEN || r < L) ? av[r] + t[r] * car[r + 1] : 0.f;
-
-    float V[kL];
-    V[0] = 1.f;
-    float G = 1.f;
-#pragma unroll
-    for (int k = 0; k < kL - 1; k++) {
-        V[k + 1] = V[k] - av[k] * G;
-        G = G * (t1 * (1 - av[k]));
-    }
-
+    // Pass 3, ranks 0..L-1: tau2/tau3 posteriors of earlier clicks, the last
+    // click, and the tau1 factors after it.
+    __asm__ volatile("" ::: "memory");
     float ex = 0.f;
 #pragma unroll
     for (int r = 0; r < kL; r++) {
@@ -420,47 +425,45 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
         const bool clicked = (mm >> r) & 1u;
         if (r < lcr) {
             if (clicked) {
-                float w2 = (1 - av[r]) * t2;
-                float w3 = av[r] * t3;
+                const float a = src.get(0, pv[r]);
+                const float w2 = (1 - a) * t2;
+                const float w3 = a * t3;
                 const float inv = __builtin_amdgcn_rcpf(w2 + w3);
-                float x = w2 * inv;
-                float y = w3 * inv;
+                const float x = w2 * inv;
+                const float y = w3 * inv;
                 ta[2] += x; ta[3] += x;
                 ta[4] += y; ta[5] += y;
             }
             continue;
         }
+        const float a = src.get(0, pv[r]);
         float Tn;
         if (VARLEN) Tn = (r + 1 < L) ? pick10(V, L - 1 - r) : 1.f;
         else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
         if (r == lcr) {
-            float cn = car[r + 1];
-            em.add(0, r, pv[r], fdiv(av[r], 1 - (t2 * (1 - av[r]) + t3 * av[r]) * cn));
-            float phi4 = (1 - av[r]) * (1 - t2);
-            float phi5 = (1 - av[r]) * t2 * Tn;
-            float phi6 = av[r] * (1 - t3);
-            float phi7 = av[r] * t3 * Tn;
-            float sum = phi4 + phi5 + phi6 + phi7;
-            const float inv = __builtin_amdgcn_rcpf(sum);
-            float n2 = phi5 * inv;
-            float n3 = phi7 * inv;
+            em.add(0, r, pv[r], fdiv(a, 1 - (t2 * (1 - a) + t3 * a) * car));
+            const float phi4 = (1 - a) * (1 - t2);
+            const float phi5 = (1 - a) * t2 * Tn;
+            const float phi6 = a * (1 - t3);
+            const float phi7 = a * t3 * Tn;
+            const float inv = __builtin_amdgcn_rcpf(phi4 + phi5 + phi6 + phi7);
+            const float n2 = phi5 * inv;
+            const float n3 = phi7 * inv;
             ta[2] += n2;
             ta[3] += n2 + phi4 * inv;
             ta[4] += n3;
             ta[5] += n3 + phi6 * inv;
-            ex = t2 * (1 - av[r]) + t3 * av[r];
+            ex = t2 * (1 - a) + t3 * a;
         } else {
-            em.add(0, r, pv[r], fdiv((1 - exam[r]) * av[r], 1 - exam[r] * car[r]));
-            float E = ex;
-            float phi0 = 1 - E;
-            float phi4 = (1 - t1) * E;
-            float phi5 = t1 * Tn * E;
-            float sum = phi0 + phi4 + phi5;
-            const float inv = __builtin_amdgcn_rcpf(sum);
-            float n1 = phi5 * inv;
+            const float E = ex;
+            const float phi0 = 1 - E;
+            const float phi4 = (1 - t1) * E;
+            const float phi5 = t1 * Tn * E;
+            const float inv = __builtin_amdgcn_rcpf(phi0 + phi4 + phi5);
+            const float n1 = phi5 * inv;
             ta[0] += n1;
             ta[1] += n1 + phi4 * inv;
-            ex = ex * fdiv(t1 * (1 - av[r]), 1 - av[r] * ex);
+            ex = ex * fdiv(t1 * (1 - a), 1 - a * ex);
         }
     }
 }
