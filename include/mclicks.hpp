@@ -217,6 +217,29 @@ public:
         return b;
     }
 
+    // Direct maximum-likelihood fitting beside EM (no counterpart in the
+    // reference): logits from the current values, then Adam steps over the
+    // minibatches set with set_option(MC_OPT_BATCHES, ...).
+    void grad_init() { check(mc_grad_init(m_.get())); }
+    // Training log-likelihood of minibatch `batch` (-1 = all) and its
+    // gradient with respect to every table's logits (public table last).
+    double loglik_gradient(std::vector<std::vector<float>>& grad, int batch = -1) {
+        std::vector<float*> ptrs;
+        grad.resize((size_t)n_tables());
+        for (int t = 0; t < n_tables(); t++) {
+            grad[(size_t)t].resize((size_t)table_size(t));
+            ptrs.push_back(grad[(size_t)t].data());
+        }
+        double ll = 0.0;
+        check(mc_loglik_gradient(m_.get(), batch, &ll, ptrs.data()));
+        return ll;
+    }
+    void adam_configure(float lr = 0.05f, float beta1 = 0.9f, float beta2 = 0.999f, float eps = 1e-8f,
+                        bool use_prior = true) {
+        check(mc_adam_configure(m_.get(), lr, beta1, beta2, eps, use_prior ? 1 : 0));
+    }
+    void adam_steps(int n_steps, bool use_graph = true) { check(mc_adam_steps(m_.get(), n_steps, use_graph ? 1 : 0)); }
+
     // Communicate::output_parameters (communicator.cpp:415-516).
     void output_parameters(const Dataset& ds, const std::string& prefix) {
         check(mc_output_parameters(m_.get(), ds.handle(), prefix.c_str()));

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime_api.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -46,6 +47,9 @@ struct Config {
     double test_share = 0.2;
     bool help = false;
     bool autotune = false;
+    std::string fit = "em";
+    double lr = 0.05;
+    int batches = 16;
 };
 
 void show_help() {
@@ -64,6 +68,10 @@ void show_help() {
         "  -e, --exec-mode <id>       0 GPU-only (this build)\n"
         "      --reduce-mode <id>     0 deterministic segmented (default), 1 float atomics\n"
         "      --autotune <0|1>       measure and pick the EM launch options first\n"
+        "      --fit <em|adam>        plain EM (default) or Adam on the log-likelihood gradient;\n"
+        "                             with adam, -i counts epochs (every minibatch once)\n"
+        "      --lr <f>               Adam learning rate (default 0.05)\n"
+        "      --batches <n>          Adam minibatches of whole query groups (default 16)\n"
         "      --resume <file>        load a parameter checkpoint before training\n"
         "      --checkpoint <file>    save a parameter checkpoint after training\n"
         "  -h, --help                 this message\n";
@@ -96,6 +104,9 @@ int main(int argc, char** argv) {
         {"--exec-mode", "-e", [&](const char* v) { config.exec_mode = std::stoi(v); }},
         {"--reduce-mode", "", [&](const char* v) { config.reduce_mode = std::stoi(v); }},
         {"--autotune", "", [&](const char* v) { config.autotune = std::stoi(v) != 0; }},
+        {"--fit", "", [&](const char* v) { config.fit = v; }},
+        {"--lr", "", [&](const char* v) { config.lr = std::stod(v); }},
+        {"--batches", "", [&](const char* v) { config.batches = std::stoi(v); }},
         {"--resume", "", [&](const char* v) { config.checkpoint_in = v; }},
         {"--checkpoint", "", [&](const char* v) { config.checkpoint_out = v; }},
         {"--help", "-h", [&](const char*) { config.help = true; }},
@@ -136,6 +147,10 @@ int main(int argc, char** argv) {
     if (config.max_gpus != 1) {
         std::cerr << "Error: this build trains on one GPU (-g 1); the query-sharded multi-GPU design is not built."
                   << std::endl;
+        return 1;
+    }
+    if (config.fit != "em" && config.fit != "adam") {
+        std::cerr << "Error: --fit must be em or adam." << std::endl;
         return 1;
     }
     if (config.model_type < 0 || config.model_type > 3) {
@@ -208,16 +223,32 @@ int main(int argc, char** argv) {
                       << ", persistent " << opts[MC_OPT_PERSISTENT] << ", block " << opts[MC_OPT_BLOCK] << ": " << ms
                       << " ms/iteration" << std::endl;
         }
+        if (config.fit == "adam") opts[MC_OPT_BATCHES] = std::max(1, config.batches);
         mclicks::ClickModel cm(config.model_type, 0);
         for (int o = 0; o < MC_N_OPTIONS; o++) cm.set_option(o, opts[o]);
         const auto t_h2d = std::chrono::steady_clock::now();
         cm.init_parameters(ds);
         const double h2d_s = seconds_since(t_h2d);
         if (!config.checkpoint_in.empty()) cm.load(config.checkpoint_in);
-        std::cout << "\nStarting " << config.iterations << " EM parameter estimation iterations..." << std::endl;
         const int itr_len = (int)std::to_string(config.iterations).length() + 1;
         double sum_itr = 0, sum_comp = 0, sum_upd = 0;
-        for (int itr = 0; itr < config.iterations; itr++) {
+        if (config.fit == "adam") {
+            std::cout << "\nStarting " << config.iterations << " Adam epochs (" << opts[MC_OPT_BATCHES]
+                      << " minibatches, lr " << config.lr << ")..." << std::endl;
+            cm.grad_init();
+            cm.adam_configure((float)config.lr);
+        } else {
+            std::cout << "\nStarting " << config.iterations << " EM parameter estimation iterations..." << std::endl;
+        }
+        for (int itr = 0; itr < config.iterations && config.fit == "adam"; itr++) {
+            double ms = 0.0;
+            mclicks::check(mc_adam_steps_timed(cm.handle(), (int)opts[MC_OPT_BATCHES], &ms));
+            sum_itr += ms / 1e3;
+            sum_comp += ms / 1e3;
+            std::cout << "Epoch: " << std::left << std::setw(itr_len) << itr << " Epoch-time: " << std::setw(11)
+                      << ms / 1e3 << std::right << std::endl;
+        }
+        for (int itr = 0; itr < config.iterations && config.fit == "em"; itr++) {
             double ms_total = 0, ms_e = 0, ms_m = 0;
             mclicks::check(mc_em_iterations_timed(cm.handle(), 1, &ms_total, &ms_e, &ms_m));
             sum_itr += ms_total / 1e3;

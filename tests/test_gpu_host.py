@@ -90,3 +90,24 @@ def test_checkpoint_resume_equals_an_uninterrupted_run(model, tmp_path):
         x.init_from_dataset(ds)
         with pytest.raises(MCError):
             check(lib().mc_load_parameters(x._h, ck.encode()))
+
+
+def test_cli_fits_with_adam(tmp_path):
+    """`--fit adam`: Adam epochs on the log-likelihood gradient instead of EM
+    iterations; the held-out LL ends within 1% of the oracle's EM result on
+    the same log."""
+    s = synth.generate(synth.config_pbm_40k())
+    path = str(tmp_path / "pbm40k.txt")
+    synth.write_raw_log(s, path)
+    print("case: cli adam", flush=True)
+    r = subprocess.run([CLI, "-r", path, "-s", "40000", "-i", "30", "-m", "0", "--fit", "adam", "--batches", "4",
+                        "--lr", "0.1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Epoch: 29" in r.stdout and "Adam epochs" in r.stdout
+    llh = float(re.search(r"Total Log likelihood is: (\S+)", r.stdout).group(1))
+    ds = oracle.parse_raw(path, 40000, 0.2)
+    tr, te = ds["train"], ds["test"]
+    om = oracle.OracleModel(PBM, tr["query"], tr["pidx"], tr["clicks"], ds["n_qd"], accum_fp64=True)
+    om.iterate(50)
+    ll_em = om.evaluate(te["pidx"], te["clicks"])["llh_sum_f64"] / len(te["query"])
+    assert abs(llh - ll_em) <= 0.01 * abs(ll_em), (llh, ll_em)
