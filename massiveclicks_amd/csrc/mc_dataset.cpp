@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mc_internal.h"
@@ -264,81 +265,180 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
     return true;
 }
 
+namespace {
+
+// One parsed line of the log.  Lines that do not start with an integer
+// (and empty lines) produce no event; any other line updates the previous
+// session id (dataset.cpp:288).
+struct LineEvent {
+    enum Kind : uint8_t { kQuery = 0, kClick = 1, kOther = 2, kError = 3 };
+    uint8_t kind;
+    int32_t sid;
+    int32_t value;      // query id (kQuery), clicked document (kClick), error code (kError)
+    int64_t line;       // line number within the chunk (for errors)
+    int64_t docs;       // index of the 10 documents in the chunk's doc list (kQuery)
+};
+
+enum { kErrSid = 1, kErrQuery = 2, kErrDoc = 3, kErrClick = 4 };
+
+struct ChunkEvents {
+    std::vector<LineEvent> ev;
+    std::vector<int32_t> docs;
+    int64_t lines = 0;
+};
+
+// Tokenises lines [p, end) (std::istringstream >> std::string per line) into
+// events; stops at the first malformed id, which the in-order pass reports
+// if the read gets that far.
+void tokenize_chunk(const char* p, const char* end, ChunkEvents* out) {
+    const char* tb[kQueryTokens + 1];
+    const char* te[kQueryTokens + 1];
+    int64_t line = 0;
+    while (p < end) {
+        const char* nl = (const char*)std::memchr(p, '\n', (size_t)(end - p));
+        const char* le = nl ? nl : end;
+        line++;
+        int nt = 0;
+        const char* t = p;
+        while (t < le) {
+            while (t < le && is_space(*t)) t++;
+            if (t >= le) break;
+            const char* s0 = t;
+            while (t < le && !is_space(*t)) t++;
+            if (nt <= kQueryTokens) { tb[nt] = s0; te[nt] = t; }
+            nt++;
+        }
+        p = nl ? nl + 1 : end;
+        if (nt == 0) continue;
+        int sid;
+        const IntParse ps = parse_int(tb[0], te[0], &sid);
+        if (ps == kIntInvalid) continue;                       // dataset.cpp:258-263
+        LineEvent e{LineEvent::kOther, sid, 0, line, -1};
+        if (ps == kIntRange) {
+            e.kind = LineEvent::kError;
+            e.value = kErrSid;
+            out->ev.push_back(e);
+            break;
+        }
+        if (nt == kQueryTokens) {
+            int q;
+            if (parse_int(tb[3], te[3], &q) != kIntOk) {
+                e.kind = LineEvent::kError;
+                e.value = kErrQuery;
+                out->ev.push_back(e);
+                break;
+            }
+            const int64_t at = (int64_t)out->docs.size();
+            bool ok = true;
+            for (int r = 0; r < kMaxSerp && ok; r++) {
+                int d;
+                ok = parse_int(tb[r + 5], te[r + 5], &d) == kIntOk;
+                out->docs.push_back(d);
+            }
+            if (!ok) {
+                e.kind = LineEvent::kError;
+                e.value = kErrDoc;
+                out->ev.push_back(e);
+                break;
+            }
+            e.kind = LineEvent::kQuery;
+            e.value = q;
+            e.docs = at;
+        } else if (nt == kClickTokens) {
+            int d;
+            if (parse_int(tb[3], te[3], &d) != kIntOk) {
+                e.kind = LineEvent::kError;
+                e.value = kErrClick;
+                out->ev.push_back(e);
+                break;
+            }
+            e.kind = LineEvent::kClick;
+            e.value = d;
+        }
+        out->ev.push_back(e);
+    }
+    out->lines = line;
+}
+
+}  // namespace
+
+// Reads the raw log (dataset.cpp:224-305): lines are tokenised in parallel
+// chunks (cut at line boundaries), then the SERP state machine runs over the
+// events in file order, so the result is that of the sequential read.
 bool parse_raw_log(const char* path, int max_sessions, std::vector<int32_t>* q_all, std::vector<int32_t>* d_all,
                    std::vector<uint8_t>* c_all, std::string* err) {
     FILE* f = std::fopen(path, "rb");
     if (!f) { *err = std::string("cannot open ") + path; return false; }
     std::string buf;
     {
+        if (std::fseek(f, 0, SEEK_END) == 0) {
+            const long size = std::ftell(f);
+            if (size > 0) buf.reserve((size_t)size);
+            std::fseek(f, 0, SEEK_SET);
+        }
         char chunk[1 << 16];
         size_t got;
         while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.append(chunk, got);
         std::fclose(f);
     }
-    const char* p = buf.data();
-    const char* end = p + buf.size();
+    const char* base = buf.data();
+    const char* end = base + buf.size();
+
+    // Chunks of at least 4 MB, at most 16 threads.
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t n_chunks = std::max<size_t>(1, std::min<size_t>(std::min<size_t>(hw, 16), buf.size() / (4u << 20)));
+    std::vector<const char*> cut(n_chunks + 1, end);
+    cut[0] = base;
+    for (size_t c = 1; c < n_chunks; c++) {
+        const char* q = base + buf.size() * c / n_chunks;
+        if (q < cut[c - 1]) q = cut[c - 1];
+        const char* nl = (const char*)std::memchr(q, '\n', (size_t)(end - q));
+        cut[c] = nl ? nl + 1 : end;
+    }
+    std::vector<ChunkEvents> chunks(n_chunks);
+    {
+        std::vector<std::thread> th;
+        for (size_t c = 1; c < n_chunks; c++) th.emplace_back(tokenize_chunk, cut[c], cut[c + 1], &chunks[c]);
+        tokenize_chunk(cut[0], cut[1], &chunks[0]);
+        for (auto& x : th) x.join();
+    }
 
     int32_t cur_q = -1, cur_d[kMaxSerp];
     uint8_t cur_c[kMaxSerp];
     bool have_cur = false;
     int n_queries = 0, sid_prev = -1;
-    const char* tb[kQueryTokens + 1];
-    const char* te[kQueryTokens + 1];
-    long line_no = 0;
-
-    while (p < end && (n_queries < max_sessions || max_sessions == -1)) {
-        const char* nl = (const char*)std::memchr(p, '\n', (size_t)(end - p));
-        const char* le = nl ? nl : end;
-        line_no++;
-        // Tokenise (std::istringstream >> std::string).
-        int nt = 0;
-        const char* t = p;
-        while (t < le) {
-            while (t < le && is_space(*t)) t++;
-            if (t >= le) break;
-            const char* s = t;
-            while (t < le && !is_space(*t)) t++;
-            if (nt <= kQueryTokens) { tb[nt] = s; te[nt] = t; }
-            nt++;
-        }
-        p = nl ? nl + 1 : end;
-        if (nt == 0) continue;
-        int sid;
-        IntParse ps = parse_int(tb[0], te[0], &sid);
-        if (ps == kIntInvalid) continue;                      // dataset.cpp:258-263
-        if (ps == kIntRange) { *err = "session id out of range at line " + std::to_string(line_no); return false; }
-        if (nt == kQueryTokens) {
-            if (sid_prev != -1 && have_cur) {                   // dataset.cpp:272-275
-                q_all->push_back(cur_q);
-                d_all->insert(d_all->end(), cur_d, cur_d + kMaxSerp);
-                c_all->insert(c_all->end(), cur_c, cur_c + kMaxSerp);
-                n_queries++;
-            }
-            if (parse_int(tb[3], te[3], &cur_q) != kIntOk) {
-                *err = "bad query id at line " + std::to_string(line_no);
+    int64_t line_base = 0;
+    for (size_t c = 0; c < n_chunks; c++) {
+        const ChunkEvents& ch = chunks[c];
+        for (const LineEvent& e : ch.ev) {
+            if (!(n_queries < max_sessions || max_sessions == -1)) return true;   // dataset.cpp:250
+            if (e.kind == LineEvent::kError) {
+                static const char* what[] = {"", "session id out of range", "bad query id", "bad document id",
+                                             "bad clicked document id"};
+                *err = std::string(what[e.value]) + " at line " + std::to_string(line_base + e.line);
                 return false;
             }
-            for (int r = 0; r < kMaxSerp; r++) {
-                if (parse_int(tb[r + 5], te[r + 5], &cur_d[r]) != kIntOk) {
-                    *err = "bad document id at line " + std::to_string(line_no);
-                    return false;
+            if (e.kind == LineEvent::kQuery) {
+                if (sid_prev != -1 && have_cur) {                   // dataset.cpp:272-275
+                    q_all->push_back(cur_q);
+                    d_all->insert(d_all->end(), cur_d, cur_d + kMaxSerp);
+                    c_all->insert(c_all->end(), cur_c, cur_c + kMaxSerp);
+                    n_queries++;
                 }
-                cur_c[r] = 0;
-            }
-            have_cur = true;
-        } else if (nt == kClickTokens) {
-            int doc;
-            if (parse_int(tb[3], te[3], &doc) != kIntOk) {
-                *err = "bad clicked document id at line " + std::to_string(line_no);
-                return false;
-            }
-            if (have_cur) {
+                cur_q = e.value;
                 for (int r = 0; r < kMaxSerp; r++) {
-                    if (cur_d[r] == doc) { cur_c[r] = 1; break; }
+                    cur_d[r] = ch.docs[(size_t)e.docs + r];
+                    cur_c[r] = 0;
+                }
+                have_cur = true;
+            } else if (e.kind == LineEvent::kClick && have_cur) {
+                for (int r = 0; r < kMaxSerp; r++) {
+                    if (cur_d[r] == e.value) { cur_c[r] = 1; break; }
                 }
             }
+            sid_prev = e.sid;                                       // dataset.cpp:288
         }
-        sid_prev = sid;                                          // dataset.cpp:288
+        line_base += ch.lines;
     }
     return true;
 }

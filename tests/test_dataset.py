@@ -188,3 +188,49 @@ def test_length_errors():
     with pytest.raises(MCError) as e:
         Dataset.from_sessions(q, docs, clicks, 0.2, length=np.array([10, 5], np.uint8))
     assert "past len" in str(e.value)
+
+
+def _noisy_log(tmp_path, n_sessions, seed, error_at=None):
+    """A synthetic log (about 85 bytes per session) with malformed lines
+    spread through it: words, empty lines, lines of other token counts,
+    clicks on absent documents, repeated session ids."""
+    s = synth.generate(synth.SynthConfig(n_sessions=n_sessions, n_queries=n_sessions // 20,
+                                         vocab=n_sessions, model="pbm", seed=seed))
+    path = tmp_path / f"noisy{seed}.txt"
+    synth.write_raw_log(s, str(path))
+    lines = path.read_text().split("\n")
+    rng = np.random.default_rng(seed)
+    noise = ["garbage words here", "", "7 1 X 5", "12 0 Q 1 2", "3 0 C 123456789",
+             "x 0 Q 1 0 1 2 3 4 5 6 7 8 9 10", "  ", "4 0 C"]
+    for pos in sorted(rng.choice(len(lines), 2000, replace=False), reverse=True):
+        lines.insert(int(pos), noise[int(rng.integers(len(noise)))])
+    if error_at is not None:
+        lines.insert(error_at, "5 0 Q 10 0 1 2 3 4 5 6 7 8 9 bad")
+    path.write_text("\n".join(lines))
+    return str(path)
+
+
+def test_parallel_parse_of_a_large_noisy_log_matches_the_sequential_oracle(tmp_path):
+    """A 35 MB log is tokenised in several chunks cut at line boundaries; the
+    result equals the oracle's line-by-line read, with and without a
+    session cap (including caps that end inside a later chunk)."""
+    path = _noisy_log(tmp_path, 420_000, 21)
+    assert os.path.getsize(path) > 32 << 20
+    for cap in (-1, 400_000, 123_457, 5):
+        o, d = both_parse(path, cap, 0.2)
+        assert o["n_parsed"] == d.n_parsed
+        for k in ("query", "docs", "clicks", "pidx"):
+            np.testing.assert_array_equal(o["train"][k], getattr(d.train, k))
+            np.testing.assert_array_equal(o["test"][k], getattr(d.test, k))
+
+
+def test_malformed_id_is_reported_only_if_the_read_reaches_it(tmp_path):
+    """A query line whose document id is not a number: an error when the
+    session cap has not been reached before it (line number reported), none
+    when the cap stops the read earlier (dataset.cpp:250)."""
+    path = _noisy_log(tmp_path, 420_000, 22, error_at=700_000)
+    with pytest.raises(MCError) as e:
+        Dataset.parse(path, -1, 0.2)
+    assert "bad document id at line 700001" in str(e.value)
+    d = Dataset.parse(path, 1000, 0.2)
+    assert d.n_parsed == 1000
