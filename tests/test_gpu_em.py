@@ -213,3 +213,28 @@ def test_variable_length_lists_match_oracle(model, mode):
     prel = np.abs(ev["ppl_log2_sum"] - oev["ppl_log2_sum_f64"]) / np.abs(oev["ppl_log2_sum_f64"])
     assert prel.max() < TOL_LL
     np.testing.assert_array_equal(ev["rank_count"], oev["rank_count"])
+
+
+@pytest.mark.parametrize("model", [PBM, DBN], ids=["PBM", "DBN"])
+def test_sessions_not_grouped_by_query_fall_back_to_atomics(model):
+    """mc_init_parameters with the training sessions in random order (query
+    groups not contiguous): the engine falls back to uniform chunks with
+    float atomics, parameters still match the oracle; the gradient path,
+    which needs query-group items, reports that it is unsupported."""
+    from massiveclicks_amd._lib import MCError
+    ds, tabs, _ = workload(model)
+    tr = ds["train"]
+    perm = np.random.default_rng(0).permutation(len(tr["query"]))
+    shuffled = {k: tr[k][perm] for k in ("query", "docs", "clicks", "pidx")}
+    om = oracle.OracleModel(model, shuffled["query"], shuffled["pidx"], shuffled["clicks"], ds["n_qd"],
+                            accum_fp64=True)
+    om.iterate(ITERS)
+    with ClickModel(model) as cm:
+        cm.init_parameters(sset(shuffled), ds["n_qd"])
+        cm.em(ITERS)
+        for t in range(cm.n_tables()):
+            _, _, ov = om.get_table(t)
+            assert float(np.max(np.abs(cm.get_values(t) - ov))) < TOL_PARAM
+        with pytest.raises(MCError) as e:
+            cm.grad_init()
+        assert e.value.status == 6
