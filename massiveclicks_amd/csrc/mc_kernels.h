@@ -194,7 +194,8 @@ struct SrcGlobal {          // straight from HBM, absolute indices
 // --------------------------------------------------------------------------
 
 // PBM E-step, pbm.cu:191-231.  Clicked ranks contribute (1,1) to both tables
-// (constant); unclicked ranks contribute ((a-ae)/(1-ae), 1) and ((e-ae)/(1-ae), 1).
+// (constant); unclicked ranks contribute ((a-ae)/(1-ae), 1) and ((e-ae)/(1-ae), 1),
+// both through one hardware reciprocal of 1 - ae (1 ulp).
 template <class Src, class Emit>
 __device__ __forceinline__ void pbm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             const float* sh, float (&acc)[55]) {
@@ -205,8 +206,9 @@ __device__ __forceinline__ void pbm_session(const int (&pv)[kL], int L, uint32_t
             float atr = src.get(0, p);
             float ex = sh[r];
             float ae = atr * ex;
-            float na = (atr - ae) / (1 - ae);
-            float ne = (ex - ae) / (1 - ae);
+            const float inv = __builtin_amdgcn_rcpf(1 - ae);
+            float na = (atr - ae) * inv;
+            float ne = (ex - ae) * inv;
             em.add(0, r, p, na);
             acc[r] += ne;
         }
@@ -228,7 +230,12 @@ __device__ __forceinline__ void ubm_slots(uint32_t mm, int (&slot)[kL]) {
 // coincide with a later rank's slot in the same session (e.g. no click
 // before rank 2 -> slot 12; click at rank 2 -> rank 4 uses slot 10 + 2 = 12);
 // the reference writes tmp[slot][session] in rank order, so only the last
-// rank writing a slot contributes to it.
+// rank writing a slot contributes to it.  An unclicked rank r is always that
+// last writer: a later rank q shares its slot r(r+1)/2 + 9 only when q's
+// previous click c satisfies q(q+1)/2 + c = r(r+1)/2 + 9, which forces
+// c <= r, while "no click before r" and r unclicked leave no click at or
+// before r.  Only clicked ranks are ever overwritten, and their constant
+// (1,1) terms are counted on the host.
 template <class Src, class Emit>
 __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             const float* sh, float (&acc)[55]) {
@@ -237,18 +244,15 @@ __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         if (r < L && !((mm >> r) & 1u)) {
-            bool last_writer = true;
-#pragma unroll
-            for (int q = r + 1; q < kL; q++) last_writer = last_writer && !(q < L && slot[q] == slot[r]);
             const int row = r * (r + 1) / 2;
             const int p = pv[r];
             float atr = src.get(0, p);
             float ex = sh[slot[r]];
             float ae = atr * ex;
-            float na = (atr - ae) / (1 - ae);
-            float ne = (ex - ae) / (1 - ae);
+            const float inv = __builtin_amdgcn_rcpf(1 - ae);
+            float na = (atr - ae) * inv;
+            float ne = (ex - ae) * inv;
             em.add(0, r, p, na);
-            ne = last_writer ? ne : 0.f;
 #pragma unroll
             for (int j = 0; j < r; j++) acc[row + j] += (slot[r] == row + j) ? ne : 0.f;
             acc[row + kL - 1] += (slot[r] == row + kL - 1) ? ne : 0.f;
