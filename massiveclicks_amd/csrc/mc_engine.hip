@@ -490,80 +490,121 @@ __device__ __forceinline__ float pv_or_default(const float* v, int p) {
     return p != -1 ? v[p] : 0.5f;   // PARAM_DEF_NUM / PARAM_DEF_DENOM
 }
 
+// One test session.  All of its loads are issued before any arithmetic: the
+// ten indices as five 8-byte loads (rows are 40 B, so 8-byte aligned), the
+// click bytes as five 2-byte loads, then the value gathers of every shown
+// rank; the chains then run from registers, unrolled over the ten ranks with
+// r < L guards.  The arithmetic is the reference's, operation for operation.
 template <int MODEL>
 __device__ void eval_session(const EvalArgs& E, int64_t i, int L, float (&lp)[kL], float (&fp)[kL]) {
-    const int32_t* p = E.pidx + i * kL;
-    const uint8_t* c = E.clicks + i * kL;
+    int p[kL];
+    const int2* p2 = reinterpret_cast<const int2*>(E.pidx + i * kL);
+#pragma unroll
+    for (int u = 0; u < kL / 2; u++) {
+        const int2 v = p2[u];
+        p[2 * u] = v.x;
+        p[2 * u + 1] = v.y;
+    }
+    bool c[kL];
+    const uint16_t* c2 = reinterpret_cast<const uint16_t*>(E.clicks + i * kL);
+#pragma unroll
+    for (int u = 0; u < kL / 2; u++) {
+        const uint16_t v = c2[u];
+        c[2 * u] = (v & 0xffu) == 1;
+        c[2 * u + 1] = (v >> 8) == 1;
+    }
+    float a[kL], sv[kL];
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        a[r] = r < L ? pv_or_default(E.val0, p[r]) : 0.5f;
+        sv[r] = (MODEL == kDBN && r < L) ? pv_or_default(E.val1, p[r]) : 0.5f;
+    }
     if (MODEL == kPBM) {
-        for (int r = 0; r < L; r++) {
-            float atr = pv_or_default(E.val0, p[r]);
-            float ae = atr * E.sh[r];
-            lp[r] = c[r] == 1 ? logf(ae) : logf(1 - ae);
-            fp[r] = c[r] == 1 ? ae : 1 - ae;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * E.sh[r];
+                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
+                fp[r] = c[r] ? ae : 1 - ae;
+            }
         }
     } else if (MODEL == kUBM) {
         int prev[kL];
         int last = kL - 1;
-        for (int r = 0; r < L; r++) { prev[r] = last; if (c[r] == 1) last = r; }
-        for (int r = 0; r < L; r++) {
-            float atr = pv_or_default(E.val0, p[r]);
-            float ae = atr * E.sh[r * (r + 1) / 2 + prev[r]];
-            lp[r] = c[r] == 1 ? logf(ae) : logf(1 - ae);
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            prev[r] = last;
+            if (r < L && c[r]) last = r;
         }
-        float tf[kL];
-        for (int r = 0; r < L; r++) {
-            float cp = 0.f;
-            for (int rp = -1; rp < r; rp++) {
-                float nb = 1.f;
-                int crp = rp == -1 ? 9 : rp;
-                for (int rb = rp + 1; rb < r; rb++) {
-                    float av = pv_or_default(E.val0, p[rb]);
-                    float ev = E.sh[rb * (rb + 1) / 2 + crp];
-                    nb *= 1 - av * ev;
-                }
-                float av = pv_or_default(E.val0, p[r]);
-                float ev = E.sh[r * (r + 1) / 2 + crp];
-                float temp = nb * (av * ev);
-                cp += rp >= 0 ? tf[rp] * temp : temp;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * E.sh[r * (r + 1) / 2 + prev[r]];
+                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
             }
-            tf[r] = cp;
-            fp[r] = c[r] == 1 ? cp : 1 - cp;
+        }
+        // P(C_r = 1) summed over the rank of the previous click (-1: none).
+        float tf[kL];
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float cp = 0.f;
+#pragma unroll
+                for (int rp = -1; rp < r; rp++) {
+                    float nb = 1.f;
+                    const int crp = rp == -1 ? 9 : rp;
+#pragma unroll
+                    for (int rb = rp + 1; rb < r; rb++) nb *= 1 - a[rb] * E.sh[rb * (rb + 1) / 2 + crp];
+                    float temp = nb * (a[r] * E.sh[r * (r + 1) / 2 + crp]);
+                    cp += rp >= 0 ? tf[rp] * temp : temp;
+                }
+                tf[r] = cp;
+                fp[r] = c[r] ? cp : 1 - cp;
+            } else {
+                tf[r] = 0.f;
+            }
         }
     } else if (MODEL == kDBN) {
         const float g = E.sh[0];
         float ex = 1.f;
-        for (int r = 0; r < L; r++) {
-            float a = pv_or_default(E.val0, p[r]);
-            float s = pv_or_default(E.val1, p[r]);
-            float cp;
-            if (c[r] == 1) { cp = a * ex; ex = g * (1 - s); }
-            else { cp = 1 - a * ex; ex *= g * (1 - a) / cp; }
-            lp[r] = logf(cp);
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float cp;
+                if (c[r]) { cp = a[r] * ex; ex = g * (1 - sv[r]); }
+                else { cp = 1 - a[r] * ex; ex *= g * (1 - a[r]) / cp; }
+                lp[r] = logf(cp);
+            }
         }
         ex = 1.f;
-        for (int r = 0; r < L; r++) {
-            float a = pv_or_default(E.val0, p[r]);
-            float s = pv_or_default(E.val1, p[r]);
-            float ae = a * ex;
-            fp[r] = c[r] == 1 ? ae : 1 - ae;
-            ex *= g * (1 - a) + g * a * (1 - s);
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * ex;
+                fp[r] = c[r] ? ae : 1 - ae;
+                ex *= g * (1 - a[r]) + g * a[r] * (1 - sv[r]);
+            }
         }
     } else {
         const float t1 = E.sh[0], t2 = E.sh[1], t3 = E.sh[2];
         float ex = 1.f;
-        for (int r = 0; r < L; r++) {
-            float a = pv_or_default(E.val0, p[r]);
-            float cp;
-            if (c[r] == 1) { cp = a * ex; ex = t2 * (1 - a) + t3 * a; }
-            else { cp = 1 - a * ex; ex *= t1 * (1 - a) / cp; }
-            lp[r] = logf(cp);
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float cp;
+                if (c[r]) { cp = a[r] * ex; ex = t2 * (1 - a[r]) + t3 * a[r]; }
+                else { cp = 1 - a[r] * ex; ex *= t1 * (1 - a[r]) / cp; }
+                lp[r] = logf(cp);
+            }
         }
         ex = 1.f;
-        for (int r = 0; r < L; r++) {
-            float a = pv_or_default(E.val0, p[r]);
-            float ae = a * ex;
-            fp[r] = c[r] == 1 ? ae : 1 - ae;
-            ex *= (1 - a) * t1 + a * ((1 - a) * t2 + a * t3);
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * ex;
+                fp[r] = c[r] ? ae : 1 - ae;
+                ex *= (1 - a[r]) * t1 + a[r] * ((1 - a[r]) * t2 + a[r] * t3);
+            }
         }
     }
 }
@@ -587,7 +628,9 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
         float lp[kL], fp[kL];
         eval_session<MODEL>(E, i, L, lp, fp);
         float acc = 0.f;
-        for (int r = 0; r < L; r++) acc = acc + lp[r];
+#pragma unroll
+        for (int r = 0; r < kL; r++)
+            if (r < L) acc = acc + lp[r];
         const float s = acc / L;                 // evaluation.cpp:36
         v[0] += (double)s;
         if (E.llh) E.llh[i] = s;
