@@ -90,7 +90,7 @@ __device__ __forceinline__ void run_session(const int (&pv)[kL], int L, uint32_t
 // One lane walks the sessions s0 + tid, s0 + tid + blockDim, ...; the
 // indices of the next session are in flight while one is processed.  The
 // first session is passed in already loaded.
-template <int MODEL, bool VARLEN, class Src, class Emit>
+template <int MODEL, bool VARLEN, bool WIN, class Src, class Emit>
 __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const Src& src, Emit& em, const float* sh,
                                          float (&acc)[55], uint32_t m, int L, int (&pv)[kL]) {
     const int nthr = blockDim.x;
@@ -105,7 +105,7 @@ __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const
             int L3;
             int pv2[kL];
             load_head(st, s + 2 * nthr, it.s1, m3, L3);
-            load_pidx_upto(st, s + nthr, it.s1, Ln, pv2);
+            load_pidx_upto<WIN>(st, s + nthr, it.s1, Ln, pv2);
             run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
             m = mn;
             L = Ln;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const
         uint32_t m2;
         int L2;
         int pv2[kL];
-        load_session<VARLEN>(st, s + nthr, it.s1, m2, L2, pv2);
+        load_session<VARLEN, WIN>(st, s + nthr, it.s1, m2, L2, pv2);
         run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
         m = m2;
         L = L2;
@@ -141,7 +141,7 @@ __device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st
         uint32_t m2;
         int L2;
         int pv2[kL];
-        load_session<VARLEN>(st, s + nthr, it.s1, m2, L2, pv2);
+        load_session<VARLEN, false>(st, s + nthr, it.s1, m2, L2, pv2);
         EmitRegs em;
         em.clear();
         run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     uint32_t m0;
     int L0;
     int pv0[kL];
-    load_session<VARLEN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    load_session<VARLEN, win>(st, it.s0 + tid, it.s1, m0, L0, pv0);
     for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
     if (win) {
         for (int i = tid; i < W; i += nthr) {
@@ -222,20 +222,20 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     const SrcGlobal gsrc{A.tab[0].val_in, A.tab[1].val_in};
     if (PATH == kPathAtomic) {
         EmitGlobalFloat em{A.tab[0].gaccf, A.tab[1].gaccf};
-        run_item<MODEL, VARLEN>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
+        run_item<MODEL, VARLEN, false>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLds) {
         EmitLds<T> em{lacc};
         const SrcLds<T> lsrc{lval};
-        run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+        run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLdsFloat) {
         EmitLdsFloat<T> em{facc};
         const SrcLds<T> lsrc{lval};
-        run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+        run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathDirect) {
         run_item_direct<MODEL, VARLEN>(A, st, it, gsrc, sh, acc, m0, L0, pv0);
     } else {
         EmitGlobalFix em{A.tab[0].gacc, A.tab[1].gacc};
-        run_item<MODEL, VARLEN>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
+        run_item<MODEL, VARLEN, false>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
     }
 
     // Shared tables: fixed point, wave sums, across waves, one atomic per
@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
     uint32_t m0;
     int L0;
     int pv0[kL];
-    load_session<VARLEN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    load_session<VARLEN, true>(st, it.s0 + tid, it.s1, m0, L0, pv0);
 
     float acc[55];
 #pragma unroll
@@ -372,10 +372,10 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
         {
             EmitLds<T> em{lacc};
             const SrcLds<T> lsrc{lval};
-            run_item<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+            run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
         }
         // First session of the next item goes out before the flush.
-        load_session<VARLEN>(st, itn.s0 + tid, itn.s1, m0, L0, pv0);
+        load_session<VARLEN, true>(st, itn.s0 + tid, itn.s1, m0, L0, pv0);
         __syncthreads();
         const bool interior = (it.flags & kItemInterior) != 0;
 #pragma unroll
@@ -733,10 +733,12 @@ void shape_of(int type, int* T, int* K, int* KACC) {
 KArgs make_args(mc_model* m, int cur, int write_num) {
     KArgs A;
     std::memset(&A, 0, sizeof(A));
+    A.pidx16 = m->d_pidx16;
     A.pidx = m->d_pidx;
     A.mask = m->d_mask;
     A.len = m->varlen ? m->d_len : nullptr;
     A.N = m->N;
+    A.n_w = m->n_w;
     A.items = m->d_items;
     A.n_items = m->n_items;
     A.mode = m->reduce_mode;
@@ -1384,24 +1386,36 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     }
 
     // ---- device layout: rank-major pidx, click masks, lengths ---------------
-    std::vector<int32_t> h_pidx((size_t)n * kMaxSerp);
+    // Window items come first in device order.  Their sessions store 16-bit
+    // indices relative to the window base (the kernels address the LDS
+    // window with them): 20 bytes per session instead of 40.  The other
+    // sessions store absolute 32-bit indices.
+    const int64_t n_w = m->n_win_items > 0 ? (int64_t)items[(size_t)m->n_win_items - 1].s1 : 0;
+    const int64_t n_rest = n - n_w;
+    m->n_w = n_w;
+    std::vector<uint16_t> h_pidx16((size_t)n_w * kMaxSerp);
+    std::vector<int32_t> h_pidx((size_t)n_rest * kMaxSerp);
     std::vector<uint16_t> h_mask((size_t)n);
     std::vector<uint8_t> h_len((size_t)n);
-    // Sessions of window items store their indices relative to the window
-    // base (the kernel addresses the LDS window with them); other sessions
-    // store absolute indices.
-    for (const Item& it : items) {
-        const int32_t off = (structured && it.width > 0) ? it.base : 0;
+    for (size_t i = 0; i < items.size(); i++) {
+        const Item& it = items[i];
+        const bool win = (int)i < m->n_win_items;
+        MC_REQUIRE(!win || (it.width > 0 && it.width <= 65536 && it.s1 <= n_w), "internal: window item layout");
+        MC_REQUIRE(win || it.s0 >= n_w, "internal: window items must come first");
         for (int64_t j = it.s0; j < it.s1; j++) {
             const int64_t s = order[j];
-            for (int r = 0; r < kMaxSerp; r++)
-                h_pidx[(size_t)r * n + j] = r < L[s] ? pidx[s * kMaxSerp + r] - off : 0;
+            for (int r = 0; r < kMaxSerp; r++) {
+                const int32_t p = r < L[s] ? pidx[s * kMaxSerp + r] : 0;
+                if (win) h_pidx16[(size_t)r * n_w + j] = r < L[s] ? (uint16_t)(p - it.base) : 0;
+                else h_pidx[(size_t)r * n_rest + (j - n_w)] = p;
+            }
             h_mask[j] = mask[s];
             h_len[j] = L[s];
         }
     }
 
-    MC_TRY(dalloc(m, &m->d_pidx, h_pidx.size() * 4));
+    MC_TRY(dalloc(m, &m->d_pidx16, std::max<size_t>(h_pidx16.size() * 2, 4)));
+    MC_TRY(dalloc(m, &m->d_pidx, std::max<size_t>(h_pidx.size() * 4, 4)));
     MC_TRY(dalloc(m, &m->d_mask, h_mask.size() * 2));
     if (m->varlen) MC_TRY(dalloc(m, &m->d_len, h_len.size()));
     MC_TRY(dalloc(m, &m->d_items, items.size() * sizeof(Item)));
@@ -1424,7 +1438,10 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     MC_TRY(dalloc(m, &m->d_sh_acc_map, sh_map.size() * 4));
     MC_TRY(dalloc(m, &m->d_sh_acc, (size_t)KACC * 8));
 
-    MC_HIP(hipMemcpyAsync(m->d_pidx, h_pidx.data(), h_pidx.size() * 4, hipMemcpyHostToDevice, m->stream));
+    if (!h_pidx16.empty())
+        MC_HIP(hipMemcpyAsync(m->d_pidx16, h_pidx16.data(), h_pidx16.size() * 2, hipMemcpyHostToDevice, m->stream));
+    if (!h_pidx.empty())
+        MC_HIP(hipMemcpyAsync(m->d_pidx, h_pidx.data(), h_pidx.size() * 4, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_mask, h_mask.data(), h_mask.size() * 2, hipMemcpyHostToDevice, m->stream));
     if (m->varlen)
         MC_HIP(hipMemcpyAsync(m->d_len, h_len.data(), h_len.size(), hipMemcpyHostToDevice, m->stream));
@@ -1766,7 +1783,8 @@ mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, s
     MC_REQUIRE(bytes && model_type >= 0 && model_type <= 3 && n >= 0 && n_qd >= 0, "invalid argument");
     int T, K, KACC;
     shape_of(model_type, &T, &K, &KACC);
-    // sessions: 40 B pidx + 2 B mask; per table entry: 2 values, 2 numerators,
+    // sessions: at most 40 B pidx (20 B for sessions of window items) + 2 B
+    // mask; per table entry: 2 values, 2 numerators,
     // (base, den) and an 8-byte accumulator; shared partials per ~1k sessions.
     size_t b = (size_t)n * 42 + (size_t)n_qd * T * (16 + 8 + 8) + (size_t)K * 32;
     b += (size_t)KACC * 8 * (size_t)(n / 1024 + 1) + (size_t)(n / 1024 + 1) * sizeof(Item);

@@ -39,10 +39,11 @@ constexpr float kLogitMax = 13.8f;   // sigmoid(13.8) = 1 - 1.0e-6, the Param::v
 constexpr int kShAccStride = 64;     // shared-table sums of one step (>= 55 accumulators)
 
 struct GArgs {
+    const uint16_t* pidx16;
     const int32_t* pidx;
     const uint16_t* mask;
     const uint8_t* len;
-    int64_t N;
+    int64_t N, n_w;
     const Item* items;
     const int32_t* lists;
     float* val[2];                 // per-qd values (read; written by Adam)
@@ -342,15 +343,17 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
     float* lval = (float*)(lds + T * G.max_width);
 
     KArgs ka;
+    ka.pidx16 = G.pidx16;
     ka.pidx = G.pidx;
     ka.mask = G.mask;
     ka.len = G.len;
     ka.N = G.N;
+    ka.n_w = G.n_w;
     const Stream st = make_stream(ka);
     uint32_t m0;
     int L0;
     int pv0[kL];
-    load_session<VARLEN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    load_session<VARLEN, WIN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
     for (int k = tid; k < K; k += NT) sh[k] = G.sh_val[k];
     if (ADAM && G.prev_batch >= 0) {
         // The previous step's finalisation: every workgroup computes the
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
             uint32_t m2;
             int L2;
             int pv2[kL];
-            load_session<VARLEN>(st, s + NT, it.s1, m2, L2, pv2);
+            load_session<VARLEN, WIN>(st, s + NT, it.s1, m2, L2, pv2);
             if (WIN) grad_session<MODEL, VARLEN, !ADAM>(pv0, L0, m0, lsrc, el, sh, acc, ll);
             else grad_session<MODEL, VARLEN, !ADAM>(pv0, L0, m0, gsrc, eg, sh, acc, ll);
             m0 = m2;
@@ -590,10 +593,12 @@ mc_status ensure_grad_buffers(mc_model* m) {
 GArgs make_gargs(mc_model* m, int batch) {
     GArgs G;
     std::memset(&G, 0, sizeof G);
+    G.pidx16 = m->d_pidx16;
     G.pidx = m->d_pidx;
     G.mask = m->d_mask;
     G.len = m->varlen ? m->d_len : nullptr;
     G.N = m->N;
+    G.n_w = m->n_w;
     G.items = m->d_items;
     G.lists = m->d_lists;
     for (int t = 0; t < 2; t++) {

@@ -54,10 +54,16 @@ struct TableArgs {
 };
 
 struct KArgs {
-    const int32_t* pidx;      // [kL][N] rank-major (N * 40 bytes < 2^32)
+    // Query-document indices, rank-major.  The sessions of window items come
+    // first in device order (positions [0, n_w)) and store 16-bit indices
+    // relative to their item's window; the others (positions [n_w, N)) store
+    // absolute 32-bit indices.
+    const uint16_t* pidx16;   // [kL][n_w]
+    const int32_t* pidx;      // [kL][N - n_w] ((N - n_w) * 40 bytes < 2^32)
     const uint16_t* mask;     // [N] click bits (bit r = rank r)
     const uint8_t* len;       // [N] ranks shown, or nullptr
     int64_t N;
+    int64_t n_w;
     const Item* items;
     int32_t n_items;
     int32_t mode;             // 0 segmented, 1 float atomics
@@ -487,33 +493,46 @@ template <> struct Shape<kCCM> { static constexpr int T = 1, K = 3, KACC = 6; };
 // address arithmetic per load) and hardware bounds checks (a prefetch past
 // the last session reads zeros).
 struct Stream {
-    __amdgpu_buffer_rsrc_t pidx, mask, len;
-    uint32_t row_bytes;
+    __amdgpu_buffer_rsrc_t pidx16, pidx, mask, len;
+    uint32_t row16_bytes, row_bytes, n_w;
 };
 
 __device__ __forceinline__ Stream make_stream(const KArgs& A) {
     constexpr int kRawDwordFlags = 0x00020000;   // raw 32-bit buffer, no format conversion
     Stream st;
-    st.pidx = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx, 0, (int)(uint32_t)(A.N * 40), kRawDwordFlags);
+    const uint32_t n_rest = (uint32_t)(A.N - A.n_w);
+    st.pidx16 = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx16, 0, (int)(uint32_t)(A.n_w * 20), kRawDwordFlags);
+    st.pidx = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx, 0, (int)(n_rest * 40u), kRawDwordFlags);
     st.mask = __builtin_amdgcn_make_buffer_rsrc((void*)A.mask, 0, (int)(uint32_t)(A.N * 2), kRawDwordFlags);
     st.len = A.len ? __builtin_amdgcn_make_buffer_rsrc((void*)A.len, 0, (int)(uint32_t)A.N, kRawDwordFlags) : st.mask;
-    st.row_bytes = (uint32_t)(A.N * 4);
+    st.row16_bytes = (uint32_t)(A.n_w * 2);
+    st.row_bytes = n_rest * 4u;
+    st.n_w = (uint32_t)A.n_w;
     return st;
 }
 
+// Index of rank r of the session at device position o: a 16-bit window
+// offset for window items (WIN), else an absolute 32-bit index.
+template <bool WIN>
+__device__ __forceinline__ int load_index(const Stream& st, uint32_t o, int r) {
+    if (WIN) return (int)__builtin_amdgcn_raw_buffer_load_b16(st.pidx16, o * 2u, r * st.row16_bytes, 0);
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, (o - st.n_w) * 4u, r * st.row_bytes, 0);
+}
+
 // Loads one session's click bits, length and query-document indices
-// (rank-major layout: rank r of session s at pidx[r * N + s]).
+// (rank-major layout: rank r of the session at position s is element
+// r * rows + s of its stream).
 // Lanes past the end of their item (s >= s_end) re-read the item's last
 // session, whose cache lines are already fetched: no branch, no extra DRAM
-// traffic, and the values are never used.  s_end <= 0 (no item) reads past
-// the buffer, which returns zeros.
-template <bool VARLEN>
+// traffic, and the values are never used.  Lanes without an item (s_end <= 0)
+// compute a wrapped offset: whatever such a load returns is never used.
+template <bool VARLEN, bool WIN>
 __device__ __forceinline__ void load_session(const Stream& st, int s, int s_end, uint32_t& m, int& L, int (&pv)[kL]) {
     const uint32_t o = (uint32_t)min(s, s_end - 1);
     m = __builtin_amdgcn_raw_buffer_load_b16(st.mask, o * 2u, 0, 0);
     L = VARLEN ? (int)__builtin_amdgcn_raw_buffer_load_b8(st.len, o, 0, 0) : kL;
 #pragma unroll
-    for (int r = 0; r < kL; r++) pv[r] = (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, o * 4u, r * st.row_bytes, 0);
+    for (int r = 0; r < kL; r++) pv[r] = load_index<WIN>(st, o, r);
 }
 
 // Click bits and length of one session (index clamped as in load_session).
@@ -526,12 +545,13 @@ __device__ __forceinline__ void load_head(const Stream& st, int s, int s_end, ui
 // Query-document indices of the first L ranks of one session; ranks past L
 // are not read (a wavefront whose sessions all end before rank r issues no
 // load for it).
+template <bool WIN>
 __device__ __forceinline__ void load_pidx_upto(const Stream& st, int s, int s_end, int L, int (&pv)[kL]) {
     const uint32_t o = (uint32_t)min(s, s_end - 1);
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         pv[r] = 0;
-        if (r < L) pv[r] = (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, o * 4u, r * st.row_bytes, 0);
+        if (r < L) pv[r] = load_index<WIN>(st, o, r);
     }
 }
 

@@ -54,7 +54,9 @@ struct mc_model {
     bool varlen = false;
     bool initialized = false;
     // device buffers
-    int32_t* d_pidx = nullptr;
+    uint16_t* d_pidx16 = nullptr;   // [10][n_w] window-relative indices of window items' sessions
+    int32_t* d_pidx = nullptr;      // [10][N - n_w] absolute indices of the other sessions
+    int64_t n_w = 0;                // sessions in window items (device positions [0, n_w))
     uint16_t* d_mask = nullptr;
     uint8_t* d_len = nullptr;
     mc::Item* d_items = nullptr;
