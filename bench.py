@@ -123,19 +123,36 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
     }
 
 
-def calibration_copy(mib):
-    """One device-to-device copy of `mib` MiB (a plain copy kernel of known
-    size for scaling the FETCH_SIZE / WRITE_SIZE counters)."""
+def calibration_copy(mib, reps=1):
+    """Device-to-device copies of `mib` MiB: a plain copy kernel of known size
+    (for scaling the FETCH_SIZE / WRITE_SIZE counters) and the copy bandwidth
+    this device reaches, in GB/s counting bytes read plus bytes written (the
+    best of `reps` event-timed copies after one untimed one)."""
     import ctypes as C
     hip = C.CDLL("libamdhip64.so")
     a, b = C.c_void_p(), C.c_void_p()
-    nbytes = C.c_size_t(mib << 20)
+    n = mib << 20
+    nbytes = C.c_size_t(n)
     assert hip.hipMalloc(C.byref(a), nbytes) == 0 and hip.hipMalloc(C.byref(b), nbytes) == 0
     assert hip.hipMemset(a, 1, nbytes) == 0
     assert hip.hipMemcpy(b, a, nbytes, 3) == 0      # hipMemcpyDeviceToDevice
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    best = None
+    for _ in range(reps):
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, nbytes, 3, None) == 0
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        best = ms.value if best is None else min(best, ms.value)
     assert hip.hipDeviceSynchronize() == 0
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
     hip.hipFree(a)
     hip.hipFree(b)
+    return 2.0 * n / (best * 1e-3) / 1e9 if best else None
 
 
 def main():
@@ -239,6 +256,15 @@ def main():
     item_bytes = sess_bytes + 16 * n_tab * (ds.n_qd - n_shared) if segmented else sess_bytes
     iter_bytes = sess_bytes + 16 * n_tab * ds.n_qd
     achieved = item_bytes / (kt["ms_estep"] * 1e-3)
+    # What this engine's own layout moves per launch: 2 B per shown rank for
+    # sessions of window items (16-bit window-relative indices), 4 B for the
+    # others, the mask (and length), and per entry an item owns the value in
+    # and out of each table plus its 8-byte counts.
+    si = cm.stream_info()
+    n_w = si["n_window_sessions"]
+    extra = 2 + (1 if batched else 0)
+    layout_sess = n_w * (2 * mean_len + extra) + (n_train - n_w) * (4 * mean_len + extra)
+    layout_bytes = layout_sess + (8 * n_tab + 8) * (ds.n_qd - n_shared) if segmented else layout_sess
 
     ev = cm.evaluate(ds.test)
     log(f"held-out LL {ev['log_likelihood']:.6f}, perplexity {ev['perplexity']:.6f}")
@@ -284,6 +310,9 @@ def main():
                      "kernel": ("em_window_persistent_kernel" if segmented and opts.get("persistent", True)
                                 and opts.get("block", 256) == 256 else "em_item_kernel"),
                      "bytes_per_launch": item_bytes,
+                     "layout_bytes_per_launch": layout_bytes,
+                     "layout_frac": layout_bytes / (kt["ms_estep"] * 1e-3) / HBM_PEAK,
+                     "window_sessions": n_w,
                      "kernel_ms": kt["ms_estep"], "merge_kernel_ms": kt["ms_mstep"],
                      "iteration_bytes": iter_bytes,
                      "iteration_frac": iter_bytes / (ms_iter * 1e-3) / HBM_PEAK},
@@ -296,6 +325,9 @@ def main():
         out["time_to_target"] = time_to_target(args.ttt_sessions, args.ttt_batch, args.ttt_lr, args.ttt_max_epochs)
     if args.calib_mib:
         calibration_copy(args.calib_mib)
+    # The copy bandwidth this device reaches (1 GiB device-to-device copy),
+    # the practical ceiling beside the 8 TB/s peak.
+    out["roofline"]["measured_copy_GBps"] = calibration_copy(1024, reps=5)
     print(json.dumps(out), flush=True)
 
 

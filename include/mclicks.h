@@ -197,6 +197,12 @@ mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, s
 mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
                              int64_t* max_item_sessions);
 
+/* Session stream chosen at init (no counterpart in the reference, which
+ * keeps a 32-bit index per rank): sessions of window items store 16-bit
+ * window-relative indices, the others 32-bit absolute ones.  Returns the
+ * number of window-item sessions and the device bytes of all indices. */
+mc_status mc_get_stream_info(const mc_model* m, int64_t* n_window_sessions, int64_t* index_bytes);
+
 /* Number of mc_model_set_option options (indices 0 .. MC_N_OPTIONS - 1). */
 #define MC_N_OPTIONS 8
 

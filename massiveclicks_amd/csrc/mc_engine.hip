@@ -1801,6 +1801,13 @@ mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_sha
     return MC_OK;
 }
 
+mc_status mc_get_stream_info(const mc_model* m, int64_t* n_window_sessions, int64_t* index_bytes) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    if (n_window_sessions) *n_window_sessions = m->n_w;
+    if (index_bytes) *index_bytes = m->n_w * kMaxSerp * 2 + (m->N - m->n_w) * kMaxSerp * 4;
+    return MC_OK;
+}
+
 void mc_model_destroy(mc_model* m) {
     if (!m) return;
     (void)hipSetDevice(m->device);

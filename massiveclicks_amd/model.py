@@ -285,6 +285,13 @@ class ClickModel:
         check(_lib.lib().mc_get_layout_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return {"n_items": a.value, "n_shared_entries": b.value, "max_item_sessions": c.value}
 
+    def stream_info(self):
+        """Sessions of window items (16-bit window-relative indices) and the
+        device bytes of all session indices."""
+        a, b = C.c_int64(), C.c_int64()
+        check(_lib.lib().mc_get_stream_info(self._h, C.byref(a), C.byref(b)))
+        return {"n_window_sessions": a.value, "index_bytes": b.value}
+
 
 OPTION_NAMES = ("reduce_mode", "item_sessions", "item_window", "sort_by_lcr", "block", "direct", "persistent",
                 "batches")
