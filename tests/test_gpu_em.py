@@ -86,18 +86,27 @@ def test_params_match_oracle_after_50_iterations(model, mode):
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
 @pytest.mark.parametrize("opts", [dict(), dict(direct=True, item_sessions=512), dict(sort_by_lcr=False, block=128),
-                                  dict(item_window=64, item_sessions=256)],
+                                  dict(item_window=16, item_sessions=256)],
                          ids=["default", "direct", "unsorted-b128", "tiny-windows"])
 def test_layout_options_keep_parity(model, opts):
     """Direct single-session items, unsorted sessions, other workgroup sizes
-    and tiny windows (which force global-atomic slices) give the same
+    and tiny windows (16 entries: queries with more documents become
+    global-atomic slices, larger ones are split across items) give the same
     parameters within the bar."""
     ds, tabs, _ = workload(model)
+    n = len(ds["train"]["query"])
     with run_gpu(model, ds, **opts) as cm:
         lay = cm.layout_info()
+        si = cm.stream_info()
+        # Window items' sessions use 16-bit indices, the others 32-bit ones;
+        # tiny windows and direct items give a stream that mixes both.
+        nw = si["n_window_sessions"]
+        assert 0 < nw <= n and si["index_bytes"] == 20 * nw + 40 * (n - nw), si
+        if "item_window" in opts or "direct" in opts:
+            assert nw < n, si
         for t, (onum, oden, oval) in enumerate(tabs):
             err = float(np.max(np.abs(cm.get_values(t) - oval)))
-            assert err < TOL_PARAM, f"{opts} {lay} table {t}: {err}"
+            assert err < TOL_PARAM, f"{opts} {lay} {si} table {t}: {err}"
 
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
