@@ -70,6 +70,9 @@ def cpu_baseline(ds, model, n_sample, iters, threads):
                       f"{threads} threads: {dt:.3f} s/iter (E {te / iters:.3f} s, M {tm / iters:.3f} s)"}
 
 
+ADAM_ITEM_SESSIONS = 4096
+
+
 def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
     """Wall time for plain EM (50 iterations) and for Adam on the
     log-likelihood gradient to reach the held-out log-likelihood of EM's
@@ -95,7 +98,10 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
     batches = max(1, round(ds.n_train / batch_sessions))
     curve = []
     reached = None
-    with ClickModel(DBN, batches=batches) as cm:
+    # Minibatch steps have fewer sessions to spread over the chip than a full
+    # EM pass: 4096-session work items were the fastest of 1024-8192 for them
+    # (8192, the default, for EM).
+    with ClickModel(DBN, batches=batches, item_sessions=ADAM_ITEM_SESSIONS) as cm:
         cm.init_from_dataset(ds)
         cm.grad_init()
         cm.adam_configure(lr=lr)
@@ -110,7 +116,8 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
     log(f"time-to-target: EM 50 it {em_ms:.2f} ms -> LL {target:.6f}; Adam {curve[-1]}")
     return {
         "workload": f"DBN, {n_sessions} synthetic sessions ({ds.n_train} train), 1M Zipf(1.1) queries; Adam lr {lr}, "
-                    f"{batches} minibatches of whole query groups (~{batch_sessions} sessions), log-prior on",
+                    f"{batches} minibatches of whole query groups (~{batch_sessions} sessions), log-prior on, "
+                    f"{ADAM_ITEM_SESSIONS}-session work items (EM: 8192)",
         "target_heldout_ll": target,
         "threshold": "held-out LL within 1e-4 relative of EM's 50-iteration LL",
         "em_ms": em_ms,

@@ -470,21 +470,47 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
                 }
             }
         }
-        for (int i = tid + (interior ? PER * NT : 0); i < W; i += NT) {
-            const int p = it.base + i;
-            const int2 oc = interior ? G.counts[p] : make_int2(0, 0);
+        // The remaining entries, four per lane and pass, with all of a pass's
+        // loads issued before its first store.
+        constexpr int U = 4;
+        for (int i0 = tid + (interior ? PER * NT : 0); i0 < W; i0 += U * NT) {
+            int2 oc[U];
+            float lg[U][T], am[U][T], av[U][T];
 #pragma unroll
-            for (int t = 0; t < T; t++) {
-                const unsigned long long v = lacc[i * T + t];
-                if (interior) {
-                    const float g = entry_gradient<MODEL>(t, fix_to_double(v), oc, lval[i * T + t], ADAM && G.use_prior);
-                    if (ADAM) {
-                        G.val[t][p] = ad.apply(G.logit[t][p], G.am[t][p], G.av[t][p], g);
-                    } else {
-                        G.grad[t][p] = g;
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT;
+                const bool on = interior && i < W;
+                const int p = it.base + (on ? i : 0);
+                oc[u] = on ? G.counts[p] : make_int2(0, 0);
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    lg[u][t] = (ADAM && on) ? G.logit[t][p] : 0.f;
+                    am[u][t] = (ADAM && on) ? G.am[t][p] : 0.f;
+                    av[u][t] = (ADAM && on) ? G.av[t][p] : 0.f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = i0 + u * NT;
+                if (i >= W) continue;
+                const int p = it.base + i;
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const unsigned long long v = lacc[i * T + t];
+                    if (interior) {
+                        const float g = entry_gradient<MODEL>(t, fix_to_double(v), oc[u], lval[i * T + t],
+                                                              ADAM && G.use_prior);
+                        if (ADAM) {
+                            G.val[t][p] = ad.apply(lg[u][t], am[u][t], av[u][t], g);
+                            G.logit[t][p] = lg[u][t];
+                            G.am[t][p] = am[u][t];
+                            G.av[t][p] = av[u][t];
+                        } else {
+                            G.grad[t][p] = g;
+                        }
+                    } else if (v != 0ull) {
+                        atomicAdd(&G.gacc[t][p], v);
                     }
-                } else if (v != 0ull) {
-                    atomicAdd(&G.gacc[t][p], v);
                 }
             }
         }
@@ -780,6 +806,51 @@ void mark_value_only(mc_model* m) {
 }  // namespace
 }  // namespace mc
 
+namespace mc {
+namespace {
+
+// Captures the graph of one epoch (B fused steps from batch 0 and the closing
+// merge) if the current one does not start from the current buffers.
+mc_status ensure_adam_graph(mc_model* m) {
+    if (m->adam_graph && m->adam_graph_first == m->cur) return MC_OK;
+    if (m->adam_graph) {
+        (void)hipGraphExecDestroy(m->adam_graph);
+        m->adam_graph = nullptr;
+    }
+    hipGraph_t g;
+    MC_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    mc_status st = MC_OK;
+    for (int b = 0; b <= m->n_batches && st == MC_OK; b++) st = launch_fused_step(m, b);
+    hipError_t e = hipStreamEndCapture(m->stream, &g);
+    if (st != MC_OK) return st;
+    MC_HIP(e);
+    e = hipGraphInstantiate(&m->adam_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MC_HIP(e);
+    m->adam_graph_first = m->cur;
+    return MC_OK;
+}
+
+// Enqueues n_steps minibatch steps on the model's stream: whole epochs from
+// batch 0 replay the epoch graph, other steps are launched one by one.
+mc_status enqueue_adam_steps(mc_model* m, int n_steps, bool use_graph) {
+    const int B = m->n_batches;
+    int done = 0;
+    while (use_graph && m->adam_next_batch == 0 && n_steps - done >= B) {
+        MC_TRY(ensure_adam_graph(m));
+        MC_HIP(hipGraphLaunch(m->adam_graph, m->stream));
+        done += B;
+    }
+    for (; done < n_steps; done++) {
+        MC_TRY(launch_grad(m, m->adam_next_batch, true));
+        m->adam_next_batch = (m->adam_next_batch + 1) % B;
+    }
+    return MC_OK;
+}
+
+}  // namespace
+}  // namespace mc
+
 using namespace mc;
 
 extern "C" {
@@ -845,47 +916,26 @@ mc_status mc_adam_steps(mc_model* m, int n_steps, int use_graph) {
     MC_REQUIRE(m->grad_ready, "call mc_grad_init first");
     MC_REQUIRE(n_steps >= 0, "n_steps must be >= 0");
     MC_HIP(hipSetDevice(m->device));
-    const int B = m->n_batches;
-    int done = 0;
-    // Whole epochs from batch 0 replay one captured graph of B steps.
-    while (use_graph && m->adam_next_batch == 0 && n_steps - done >= B) {
-        if (!m->adam_graph || m->adam_graph_first != m->cur) {
-            if (m->adam_graph) {
-                (void)hipGraphExecDestroy(m->adam_graph);
-                m->adam_graph = nullptr;
-            }
-            hipGraph_t g;
-            MC_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-            mc_status st = MC_OK;
-            for (int b = 0; b <= B && st == MC_OK; b++) st = launch_fused_step(m, b);
-            hipError_t e = hipStreamEndCapture(m->stream, &g);
-            if (st != MC_OK) return st;
-            MC_HIP(e);
-            e = hipGraphInstantiate(&m->adam_graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            MC_HIP(e);
-            m->adam_graph_first = m->cur;
-        }
-        MC_HIP(hipGraphLaunch(m->adam_graph, m->stream));
-        done += B;
-    }
-    for (; done < n_steps; done++) {
-        MC_TRY(launch_grad(m, m->adam_next_batch, true));
-        m->adam_next_batch = (m->adam_next_batch + 1) % B;
-    }
+    MC_TRY(enqueue_adam_steps(m, n_steps, use_graph != 0));
     MC_HIP(hipStreamSynchronize(m->stream));
     mark_value_only(m);
     return MC_OK;
 }
 
+// Device time of n_steps steps: the epoch graph is captured before the first
+// event, so only the replayed work is timed (as for mc_em_graph_timed).
 mc_status mc_adam_steps_timed(mc_model* m, int n_steps, double* ms) {
     MC_REQUIRE(m && ms, "null argument");
+    MC_TRY(check_grad_support(m));
+    MC_REQUIRE(m->grad_ready, "call mc_grad_init first");
+    MC_REQUIRE(n_steps >= 0, "n_steps must be >= 0");
     MC_HIP(hipSetDevice(m->device));
+    if (m->adam_next_batch == 0 && n_steps >= m->n_batches) MC_TRY(ensure_adam_graph(m));
     hipEvent_t a, b;
     MC_HIP(hipEventCreate(&a));
     MC_HIP(hipEventCreate(&b));
     MC_HIP(hipEventRecord(a, m->stream));
-    mc_status st = mc_adam_steps(m, n_steps, 1);
+    mc_status st = enqueue_adam_steps(m, n_steps, true);
     MC_HIP(hipEventRecord(b, m->stream));
     MC_HIP(hipEventSynchronize(b));
     float t = 0.f;
@@ -893,6 +943,7 @@ mc_status mc_adam_steps_timed(mc_model* m, int n_steps, double* ms) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms = t;
+    if (st == MC_OK) mark_value_only(m);
     return st;
 }
 
