@@ -29,6 +29,11 @@
 #include <thread>
 #include <vector>
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "mc_internal.h"
 
 namespace mc {
@@ -120,42 +125,21 @@ private:
     size_t size_ = 0;
 };
 
-// Flat (query, doc) -> index hash used to look up test pairs.
-class PairIndex {
-public:
-    void init(size_t n) {
-        size_t cap = 16;
-        while (cap < n * 2) cap <<= 1;
-        keys_.assign(cap, kEmpty);
-        vals_.assign(cap, -1);
+// Runs f(a, b) over [0, n) in blocks of at least `grain` items on up to 16
+// threads (the calling thread takes the first block).
+template <class F>
+void parallel_ranges(int64_t n, int64_t grain, F f) {
+    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hw, 16), n / std::max<int64_t>(1, grain)));
+    if (nt <= 1) {
+        f((int64_t)0, n);
+        return;
     }
-    void insert(int q, int d, int idx) {
-        uint64_t key = pack(q, d);
-        size_t mask = keys_.size() - 1, h = hash(key) & mask;
-        while (keys_[h] != kEmpty && keys_[h] != key) h = (h + 1) & mask;
-        keys_[h] = key;
-        vals_[h] = idx;
-    }
-    int find(int q, int d) const {
-        uint64_t key = pack(q, d);
-        size_t mask = keys_.size() - 1, h = hash(key) & mask;
-        while (keys_[h] != kEmpty) {
-            if (keys_[h] == key) return vals_[h];
-            h = (h + 1) & mask;
-        }
-        return -1;
-    }
-
-private:
-    static constexpr uint64_t kEmpty = ~0ull;
-    static uint64_t pack(int q, int d) { return ((uint64_t)(uint32_t)q << 32) | (uint32_t)d; }
-    static size_t hash(uint64_t k) {
-        k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
-        return (size_t)k;
-    }
-    std::vector<uint64_t> keys_;
-    std::vector<int> vals_;
-};
+    std::vector<std::thread> th;
+    for (int64_t t = 1; t < nt; t++) th.emplace_back(f, n * t / nt, n * (t + 1) / nt);
+    f((int64_t)0, n / nt);
+    for (auto& x : th) x.join();
+}
 
 }  // namespace
 
@@ -190,22 +174,31 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
     ds.train_len.clear();
     ds.test_len.clear();
     if (len_all) ds.train_len.resize((size_t)n_train);
-    for (int i = 0; i < n_train; i++) {
-        int src = keys[i].pos;
-        ds.train_query[i] = q_all[src];
-        std::memcpy(&ds.train_docs[(size_t)i * kMaxSerp], &d_all[(size_t)src * kMaxSerp], kMaxSerp * 4);
-        std::memcpy(&ds.train_clicks[(size_t)i * kMaxSerp], &c_all[(size_t)src * kMaxSerp], kMaxSerp);
-        if (len_all) ds.train_len[i] = (*len_all)[(size_t)src];
-    }
+    // Gather in sorted order (random reads: split over threads).
+    parallel_ranges(n_train, 1 << 16, [&](int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; i++) {
+            const int src = keys[(size_t)i].pos;
+            ds.train_query[(size_t)i] = q_all[(size_t)src];
+            std::memcpy(&ds.train_docs[(size_t)i * kMaxSerp], &d_all[(size_t)src * kMaxSerp], kMaxSerp * 4);
+            std::memcpy(&ds.train_clicks[(size_t)i * kMaxSerp], &c_all[(size_t)src * kMaxSerp], kMaxSerp);
+            if (len_all) ds.train_len[(size_t)i] = (*len_all)[(size_t)src];
+        }
+    });
     auto train_shown = [&](int i, int r) { return !len_all || r < ds.train_len[(size_t)i]; };
 
     // First-occurrence indices; equal queries are contiguous after the sort,
     // so each query group is numbered with a group-local map.
+    // A query group's new indices are consecutive, so group g owns the
+    // entries [entry_lo[g], entry_lo[g + 1]); entry_doc records each
+    // entry's document for the test-set lookup below.
     GroupDocMap gmap;
     int next = 0;
+    std::vector<int32_t> group_query, entry_lo, entry_doc;
     for (int s = 0; s < n_train;) {
         int e = s;
         while (e < n_train && ds.train_query[e] == ds.train_query[s]) e++;
+        group_query.push_back(ds.train_query[s]);
+        entry_lo.push_back(next);
         gmap.reset((size_t)std::min(e - s, 64) * kMaxSerp);
         for (int i = s; i < e; i++) {
             for (int r = 0; r < kMaxSerp; r++) {
@@ -214,53 +207,94 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
                     continue;
                 }
                 bool ins;
-                int idx = gmap.find_or_insert(ds.train_docs[(size_t)i * kMaxSerp + r], next, &ins);
-                if (ins) next++;
+                const int d = ds.train_docs[(size_t)i * kMaxSerp + r];
+                int idx = gmap.find_or_insert(d, next, &ins);
+                if (ins) {
+                    entry_doc.push_back(d);
+                    next++;
+                }
                 ds.train_pidx[(size_t)i * kMaxSerp + r] = idx;
             }
         }
         s = e;
     }
+    entry_lo.push_back(next);
     ds.n_qd = next;
 
     // Test set: keep log order, drop sessions whose query is unseen or whose
-    // documents are all unseen.
-    PairIndex pairs;
-    pairs.init((size_t)ds.n_qd);
-    std::vector<int32_t> train_queries;
-    train_queries.reserve(1024);
-    for (int s = 0; s < n_train;) {
-        int e = s;
-        while (e < n_train && ds.train_query[e] == ds.train_query[s]) e++;
-        train_queries.push_back(ds.train_query[s]);
-        for (int i = s; i < e; i++)
-            for (int r = 0; r < kMaxSerp; r++)
-                if (train_shown(i, r))
-                    pairs.insert(ds.train_query[i], ds.train_docs[(size_t)i * kMaxSerp + r],
-                                 ds.train_pidx[(size_t)i * kMaxSerp + r]);
-        s = e;
+    // documents are all unseen.  A (query, doc) pair is looked up as the
+    // query's group (binary search; group_query is ascending) and the doc
+    // among the group's entries: a scan for small groups, a binary search in
+    // a per-group sorted copy for large ones.
+    constexpr int kScan = 32;
+    const int n_groups = (int)group_query.size();
+    std::vector<int64_t> sorted_at((size_t)n_groups + 1, 0);
+    for (int g = 0; g < n_groups; g++) {
+        const int w = entry_lo[(size_t)g + 1] - entry_lo[(size_t)g];
+        sorted_at[(size_t)g + 1] = sorted_at[(size_t)g] + (w > kScan ? w : 0);
     }
-    // train_queries is ascending (sorted train set).
+    std::vector<std::pair<int32_t, int32_t>> sorted_docs((size_t)sorted_at[(size_t)n_groups]);
+    parallel_ranges(n_groups, 4096, [&](int64_t a, int64_t b) {
+        for (int64_t g = a; g < b; g++) {
+            const int lo = entry_lo[(size_t)g], hi = entry_lo[(size_t)g + 1];
+            if (hi - lo <= kScan) continue;
+            auto* out = &sorted_docs[(size_t)sorted_at[(size_t)g]];
+            for (int k = lo; k < hi; k++) out[k - lo] = {entry_doc[(size_t)k], k};
+            std::sort(out, out + (hi - lo));
+        }
+    });
+    auto find_pair = [&](int g, int d) -> int {
+        const int lo = entry_lo[(size_t)g], hi = entry_lo[(size_t)g + 1];
+        if (hi - lo <= kScan) {
+            for (int k = lo; k < hi; k++)
+                if (entry_doc[(size_t)k] == d) return k;
+            return -1;
+        }
+        const auto* b = &sorted_docs[(size_t)sorted_at[(size_t)g]];
+        const auto* e = b + (hi - lo);
+        const auto* it = std::lower_bound(b, e, std::make_pair(d, INT32_MIN));
+        return (it != e && it->first == d) ? it->second : -1;
+    };
+    // Lookups in parallel over blocks of the test range, then the kept
+    // sessions are appended in log order.
+    const int n_tail = n - n_train;
+    std::vector<int32_t> t_pid((size_t)n_tail * kMaxSerp);
+    std::vector<uint8_t> t_keep((size_t)n_tail, 0);
+    parallel_ranges(n_tail, 1 << 14, [&](int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; k++) {
+            const int i = n_train + (int)k;
+            const int q = q_all[(size_t)i];
+            const auto git = std::lower_bound(group_query.begin(), group_query.end(), q);
+            if (git == group_query.end() || *git != q) continue;
+            const int g = (int)(git - group_query.begin());
+            int32_t* pid = &t_pid[(size_t)k * kMaxSerp];
+            bool found = false;
+            for (int r = 0; r < kMaxSerp; r++) {
+                pid[r] = shown(i, r) ? find_pair(g, d_all[(size_t)i * kMaxSerp + r]) : -1;
+                if (pid[r] != -1) found = true;
+            }
+            t_keep[(size_t)k] = found;
+        }
+    });
     ds.test_query.clear();
     ds.test_docs.clear();
     ds.test_clicks.clear();
     ds.test_pidx.clear();
-    for (int i = n_train; i < n; i++) {
-        int q = q_all[i];
-        if (!std::binary_search(train_queries.begin(), train_queries.end(), q)) continue;
-        int32_t pid[kMaxSerp];
-        bool found = false;
-        for (int r = 0; r < kMaxSerp; r++) {
-            pid[r] = shown(i, r) ? pairs.find(q, d_all[(size_t)i * kMaxSerp + r]) : -1;
-            if (pid[r] != -1) found = true;
-        }
-        if (!found) continue;
-        if (len_all) ds.test_len.push_back((*len_all)[(size_t)i]);
-        ds.test_query.push_back(q);
-        ds.test_docs.insert(ds.test_docs.end(), &d_all[(size_t)i * kMaxSerp], &d_all[(size_t)i * kMaxSerp] + kMaxSerp);
-        ds.test_clicks.insert(ds.test_clicks.end(), &c_all[(size_t)i * kMaxSerp],
-                              &c_all[(size_t)i * kMaxSerp] + kMaxSerp);
-        ds.test_pidx.insert(ds.test_pidx.end(), pid, pid + kMaxSerp);
+    size_t n_keep = 0;
+    for (int k = 0; k < n_tail; k++) n_keep += t_keep[(size_t)k];
+    ds.test_query.reserve(n_keep);
+    ds.test_docs.reserve(n_keep * kMaxSerp);
+    ds.test_clicks.reserve(n_keep * kMaxSerp);
+    ds.test_pidx.reserve(n_keep * kMaxSerp);
+    if (len_all) ds.test_len.reserve(n_keep);
+    for (int k = 0; k < n_tail; k++) {
+        if (!t_keep[(size_t)k]) continue;
+        const size_t i = (size_t)n_train + (size_t)k;
+        if (len_all) ds.test_len.push_back((*len_all)[i]);
+        ds.test_query.push_back(q_all[i]);
+        ds.test_docs.insert(ds.test_docs.end(), &d_all[i * kMaxSerp], &d_all[i * kMaxSerp] + kMaxSerp);
+        ds.test_clicks.insert(ds.test_clicks.end(), &c_all[i * kMaxSerp], &c_all[i * kMaxSerp] + kMaxSerp);
+        ds.test_pidx.insert(ds.test_pidx.end(), &t_pid[(size_t)k * kMaxSerp], &t_pid[(size_t)k * kMaxSerp] + kMaxSerp);
     }
     return true;
 }
@@ -294,6 +328,9 @@ void tokenize_chunk(const char* p, const char* end, ChunkEvents* out) {
     const char* tb[kQueryTokens + 1];
     const char* te[kQueryTokens + 1];
     int64_t line = 0;
+    // Query lines carry 15 tokens (at least ~30 bytes), click lines 4.
+    out->ev.reserve((size_t)(end - p) / 32 + 16);
+    out->docs.reserve((size_t)(end - p) / 12 + 16);
     while (p < end) {
         const char* nl = (const char*)std::memchr(p, '\n', (size_t)(end - p));
         const char* le = nl ? nl : end;
@@ -367,30 +404,45 @@ void tokenize_chunk(const char* p, const char* end, ChunkEvents* out) {
 // events in file order, so the result is that of the sequential read.
 bool parse_raw_log(const char* path, int max_sessions, std::vector<int32_t>* q_all, std::vector<int32_t>* d_all,
                    std::vector<uint8_t>* c_all, std::string* err) {
-    FILE* f = std::fopen(path, "rb");
-    if (!f) { *err = std::string("cannot open ") + path; return false; }
+    // A regular file is mapped (the tokenising threads fault its pages in
+    // parallel); anything else is read into memory.
+    struct Mapping {
+        void* p = MAP_FAILED;
+        size_t n = 0;
+        ~Mapping() { if (p != MAP_FAILED) munmap(p, n); }
+    } map;
     std::string buf;
+    const char* base = nullptr;
+    size_t size = 0;
     {
-        if (std::fseek(f, 0, SEEK_END) == 0) {
-            const long size = std::ftell(f);
-            if (size > 0) buf.reserve((size_t)size);
-            std::fseek(f, 0, SEEK_SET);
+        const int fd = open(path, O_RDONLY);
+        if (fd < 0) { *err = std::string("cannot open ") + path; return false; }
+        struct stat sb;
+        if (fstat(fd, &sb) == 0 && S_ISREG(sb.st_mode) && sb.st_size > 0) {
+            map.n = (size_t)sb.st_size;
+            map.p = mmap(nullptr, map.n, PROT_READ, MAP_PRIVATE, fd, 0);
         }
-        char chunk[1 << 16];
-        size_t got;
-        while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.append(chunk, got);
-        std::fclose(f);
+        if (map.p != MAP_FAILED) {
+            base = (const char*)map.p;
+            size = map.n;
+        } else {
+            char chunk[1 << 16];
+            ssize_t got;
+            while ((got = read(fd, chunk, sizeof(chunk))) > 0) buf.append(chunk, (size_t)got);
+            base = buf.data();
+            size = buf.size();
+        }
+        close(fd);
     }
-    const char* base = buf.data();
-    const char* end = base + buf.size();
+    const char* end = base + size;
 
     // Chunks of at least 4 MB, at most 16 threads.
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t n_chunks = std::max<size_t>(1, std::min<size_t>(std::min<size_t>(hw, 16), buf.size() / (4u << 20)));
+    const size_t n_chunks = std::max<size_t>(1, std::min<size_t>(std::min<size_t>(hw, 16), size / (4u << 20)));
     std::vector<const char*> cut(n_chunks + 1, end);
     cut[0] = base;
     for (size_t c = 1; c < n_chunks; c++) {
-        const char* q = base + buf.size() * c / n_chunks;
+        const char* q = base + size * c / n_chunks;
         if (q < cut[c - 1]) q = cut[c - 1];
         const char* nl = (const char*)std::memchr(q, '\n', (size_t)(end - q));
         cut[c] = nl ? nl + 1 : end;
@@ -403,6 +455,13 @@ bool parse_raw_log(const char* path, int max_sessions, std::vector<int32_t>* q_a
         for (auto& x : th) x.join();
     }
 
+    {
+        size_t n_query_lines = 0;
+        for (const ChunkEvents& ch : chunks) n_query_lines += ch.docs.size() / kMaxSerp;
+        q_all->reserve(n_query_lines);
+        d_all->reserve(n_query_lines * kMaxSerp);
+        c_all->reserve(n_query_lines * kMaxSerp);
+    }
     int32_t cur_q = -1, cur_d[kMaxSerp];
     uint8_t cur_c[kMaxSerp];
     bool have_cur = false;

@@ -93,6 +93,26 @@ def test_test_set_filtering():
         np.testing.assert_array_equal(o["test"][k], getattr(d.test, k))
 
 
+def test_test_lookup_in_queries_with_many_documents():
+    """Queries whose train sessions show many distinct documents (more than
+    the 32 a query group is scanned for) and few: every test pair resolves
+    to the index the oracle assigns, unseen documents to -1."""
+    rng = np.random.default_rng(11)
+    n = 4000
+    q = rng.integers(0, 6, n).astype(np.int32)
+    # query 0 and 1 draw from 400 documents, the others from 12
+    pool = np.where(q[:, None] < 2, rng.integers(0, 400, (n, 10)), rng.integers(0, 12, (n, 10)))
+    docs = (pool + 1000 * q[:, None]).astype(np.int32)
+    clicks = (rng.random((n, 10)) < 0.2).astype(np.uint8)
+    d = Dataset.from_sessions(q, docs, clicks, 0.3)
+    o = oracle.from_sessions(q, docs, clicks, 0.3)
+    assert d.n_qd == o["n_qd"] and d.n_qd > 800
+    for part in ("train", "test"):
+        for k in ("query", "docs", "pidx"):
+            np.testing.assert_array_equal(o[part][k], getattr(getattr(d, part), k))
+    assert (d.test.pidx == -1).any() and (d.test.pidx >= 0).any()
+
+
 def test_train_order_is_libstdcxx_sort_of_equal_keys():
     """The train set is ordered by std::sort on the query (dataset.cpp:19-20),
     which is not stable: the committed fixture holds the permutation
