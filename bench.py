@@ -179,8 +179,9 @@ def main():
     ap.add_argument("--item-window", type=int, default=None)
     ap.add_argument("--model", default="dbn", choices=["pbm", "ubm", "dbn", "ccm"])
     ap.add_argument("--mode", default="segmented", choices=["segmented", "atomic"])
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    # About 10 s of CPU work on 16 threads.
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--item-sessions", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
