@@ -257,7 +257,8 @@ def main():
     n_train = ds.n_train
     value = n_train / (ms_iter * 1e-3)
     n_shared = lay["n_shared_entries"]
-    # Session stream: 4 B per shown rank + 2 B click mask (+ 1 B length).
+    # Algorithmic bytes of the roofline: a 32-bit index per shown rank + 2 B
+    # click mask (+ 1 B length) per session, 16 B per entry and table.
     mean_len = float(ds.train.length.mean()) if ds.train.length is not None else 10.0
     sess_bytes = n_train * (4 * mean_len + 2 + (1 if batched else 0))
     segmented = opts.get("reduce_mode", REDUCE_SEGMENTED) == REDUCE_SEGMENTED
