@@ -296,9 +296,12 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 }
 
 // Persistent form of the window path: workgroup b of G resident ones
-// processes window items b, b + G, b + 2G, ... (items are ordered largest
-// first, so the striding balances the load; a fixed assignment keeps the
-// per-lane float sums, and so the results, identical from run to run).
+// processes one window item per round, item kG + b in even rounds k and
+// kG + G - 1 - b in odd ones.  Items are ordered largest first, so this
+// back-and-forth order pairs the larger items of one round with the smaller
+// of the next, while the items of a round stay adjacent in memory; a fixed
+// assignment keeps the per-lane float sums, and so the results, identical
+// from run to run.
 // While item k is processed, the window values of the workgroup's next item
 // and its first session are already in flight (registers), and the constants
 // of item k's entries arrive during its session loop, so the round trips of
@@ -322,7 +325,10 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
     const Stream st = make_stream(A);
     for (int k = tid; k < K; k += NT) sh[k] = A.sh_val_in[k];
 
-    int cur = blockIdx.x;
+    const int G = (int)gridDim.x;
+    auto item_of_round = [&](int k) { return k * G + ((k & 1) ? G - 1 - (int)blockIdx.x : (int)blockIdx.x); };
+    int round = 0;
+    int cur = item_of_round(0);
     Item it = cur < n_win ? A.items[cur] : Item{0, 0, 0, 0, 0, 0};
 
     // Window values of the current item, and its first session.
@@ -357,7 +363,7 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
             }
         }
         __syncthreads();
-        const int nxt = cur + (int)gridDim.x;
+        const int nxt = item_of_round(round + 1);
         const Item itn = nxt < n_win ? A.items[nxt] : Item{0, 0, 0, 0, 0, 0};
         // In flight during the session loop: next window values, this
         // window's counts.
@@ -397,6 +403,7 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
             }
         }
         __syncthreads();
+        round++;
         cur = nxt;
         it = itn;
     }
