@@ -1370,6 +1370,16 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     for (const Item& it : items) max_width = std::max(max_width, (int)it.width);
     m->max_width = max_width;
     m->lds_bytes = (size_t)max_width * T * (m->reduce_mode == MC_REDUCE_SEGMENTED ? 12 : 8);
+    {
+        // The window (plus the kernels' few KB of static LDS) must fit one
+        // workgroup's LDS.
+        int max_lds = 0;
+        MC_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, m->device));
+        MC_REQUIRE(m->lds_bytes + 4096 <= (size_t)max_lds,
+                   "item window too large for this device's LDS (" + std::to_string(max_width) + " entries need " +
+                       std::to_string(m->lds_bytes) + " bytes, at most " + std::to_string(max_lds) +
+                       " per workgroup): use a smaller MC_OPT_ITEM_WINDOW");
+    }
 
     // Persistent window kernel: 256-thread workgroups, windows staged through
     // registers (width <= 256 * kPersistPer), one workgroup per resident slot.

@@ -109,6 +109,19 @@ def test_layout_options_keep_parity(model, opts):
             assert err < TOL_PARAM, f"{opts} {lay} {si} table {t}: {err}"
 
 
+def test_window_larger_than_lds_is_rejected():
+    """Windows of 16384 entries (12 bytes each for one table) do not fit a
+    workgroup's LDS: init reports it instead of failing at launch."""
+    ds, _, _ = workload(PBM)
+    assert ds["n_qd"] > 16384
+    cm = ClickModel(PBM, device=0, item_window=16384, item_sessions=1 << 20)
+    try:
+        with pytest.raises(RuntimeError, match="item window too large"):
+            cm.init_parameters(sset(ds["train"]), ds["n_qd"])
+    finally:
+        cm.close()
+
+
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
 def test_params_match_reference_faithful_oracle(model):
     ds, tabs, _ = workload(model, small=True, accum_fp64=False)
