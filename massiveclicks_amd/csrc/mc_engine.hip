@@ -267,22 +267,23 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     if (win && !fwin) {
         const bool interior = (it.flags & kItemInterior) != 0;
         for (int i0 = 0; i0 < W; i0 += 4 * nthr) {
-            int2 oc[4];
+            uint32_t pk[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int i = i0 + u * nthr + tid;
-                oc[u] = (interior && i < W) ? A.counts[it.base + i] : make_int2(0, 0);
+                pk[u] = (interior && i < W) ? A.counts16[it.base + i] : 0u;
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int i = i0 + u * nthr + tid;
                 if (i >= W) continue;
                 const int p = it.base + i;
+                const int2 oc = interior ? unpack_counts(pk[u], A.counts, p) : make_int2(0, 0);
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     const unsigned long long v = lacc[i * T + t];
                     if (interior) {
-                        const float2 bd = base_den<MODEL>(t, oc[u]);
+                        const float2 bd = base_den<MODEL>(t, oc);
                         const float num = fix_to_float(bd.x, v);
                         A.tab[t].val_out[p] = clamp_value(num, bd.y);
                         if (A.write_num) A.tab[t].num_out[p] = num;
@@ -367,13 +368,13 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
         const Item itn = nxt < n_win ? A.items[nxt] : Item{0, 0, 0, 0, 0, 0};
         // In flight during the session loop: next window values, this
         // window's counts.
-        int2 cnt[PER];
+        uint32_t cnt[PER];
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int i = u * NT + tid;
 #pragma unroll
             for (int t = 0; t < T; t++) wv[u][t] = (i < itn.width) ? A.tab[t].val_in[itn.base + i] : 0.f;
-            cnt[u] = (i < W) ? A.counts[it.base + i] : make_int2(0, 0);
+            cnt[u] = (i < W) ? A.counts16[it.base + i] : 0u;
         }
         {
             EmitLds<T> em{lacc};
@@ -389,11 +390,12 @@ __global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int 
             const int i = u * NT + tid;
             if (i >= W) continue;
             const int p = it.base + i;
+            const int2 oc = interior ? unpack_counts(cnt[u], A.counts, p) : make_int2(0, 0);
 #pragma unroll
             for (int t = 0; t < T; t++) {
                 const unsigned long long v = lacc[i * T + t];
                 if (interior) {
-                    const float2 bd = base_den<MODEL>(t, cnt[u]);
+                    const float2 bd = base_den<MODEL>(t, oc);
                     const float num = fix_to_float(bd.x, v);
                     A.tab[t].val_out[p] = clamp_value(num, bd.y);
                     if (A.write_num) A.tab[t].num_out[p] = num;
@@ -765,6 +767,7 @@ KArgs make_args(mc_model* m, int cur, int write_num) {
     A.n_tables = m->T;
     A.write_num = write_num;
     A.counts = m->d_counts;
+    A.counts16 = m->d_counts16;
     A.max_width = m->max_width;
     A.K = m->K;
     A.kacc = m->KACC;
@@ -1451,6 +1454,7 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
         MC_TRY(dalloc(m, &m->d_sh_den[b], (size_t)K * 4));
     }
     MC_TRY(dalloc(m, &m->d_counts, (size_t)n_qd * 8));
+    MC_TRY(dalloc(m, &m->d_counts16, (size_t)n_qd * 2));
     MC_TRY(dalloc(m, &m->d_sh_base, sh_base.size() * 8));
     MC_TRY(dalloc(m, &m->d_sh_acc_map, sh_map.size() * 4));
     MC_TRY(dalloc(m, &m->d_sh_acc, (size_t)KACC * 8));
@@ -1469,6 +1473,11 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     if (!lists.empty())
         MC_HIP(hipMemcpyAsync(m->d_lists, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_counts, counts.data(), counts.size() * 8, hipMemcpyHostToDevice, m->stream));
+    std::vector<uint16_t> counts16((size_t)n_qd);
+    for (int64_t p = 0; p < n_qd; p++)
+        counts16[(size_t)p] = counts[(size_t)p].x < 255 ? (uint16_t)(counts[(size_t)p].x | (counts[(size_t)p].y << 8))
+                                                      : (uint16_t)kCountsEscape;
+    MC_HIP(hipMemcpyAsync(m->d_counts16, counts16.data(), counts16.size() * 2, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_sh_base, sh_base.data(), sh_base.size() * 8, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_sh_acc_map, sh_map.data(), sh_map.size() * 4, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemsetAsync(m->d_sh_acc, 0, (size_t)KACC * 8, m->stream));

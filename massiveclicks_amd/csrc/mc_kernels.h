@@ -69,6 +69,7 @@ struct KArgs {
     int32_t mode;             // 0 segmented, 1 float atomics
     TableArgs tab[2];
     const int2* counts;       // per qd entry: (occurrences, clicked occurrences) in training
+    const uint16_t* counts16; // the same packed as o | c << 8 when o < 255, else kCountsEscape
     const float* sh_val_in;   // shared table values of theta_t
     unsigned long long* sh_acc;   // [kacc] fixed-point accumulators of the shared tables
     int32_t max_width;            // largest item window (LDS sizing)
@@ -87,6 +88,13 @@ struct KArgs {
     float* sh_num_out;
     float* sh_den_out;
 };
+
+// Counts of an entry from its 2-byte packed form, or from the full array
+// when they do not fit (o >= 255; clicked occurrences never exceed o).
+constexpr uint32_t kCountsEscape = 0xFFFFu;
+__device__ __forceinline__ int2 unpack_counts(uint32_t pk, const int2* counts, int p) {
+    return pk != kCountsEscape ? make_int2((int)(pk & 0xFFu), (int)(pk >> 8)) : counts[p];
+}
 
 // Parameter-independent part of the M-step of per-qd table t for an entry
 // with `o` occurrences of which `c` were clicked: (prior + constant expected

@@ -71,6 +71,7 @@ struct mc_model {
     float* d_val[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [buf][t]
     float* d_num[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     int2* d_counts = nullptr;
+    uint16_t* d_counts16 = nullptr;  // packed counts read by the window flushes
     std::vector<int2> h_counts;   // host copy (occurrences, clicked) per entry
     unsigned long long* d_gacc[2] = {nullptr, nullptr};
     float* d_den_user[2] = {nullptr, nullptr};
