@@ -309,8 +309,16 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 // staging and flushing overlap with the E-step.  The shared-table counts stay
 // in registers across items and are reduced once per workgroup.  Requires
 // blockDim.x == 256 and width <= 256 * PER.
+// Waves per SIMD the persistent kernel asks the compiler to fit: ten-rank DBN
+// and CCM go from 4 waves (108 and 114 VGPRs) to 5 (96 VGPRs, spilling
+// 5-8 dwords), which measured 1-3 % faster; PBM already fits 6 (7 measured
+// slower) and UBM's 55 accumulators leave no room.
+template <int MODEL, bool VARLEN>
+struct PersistWaves { static constexpr int value = (!VARLEN && (MODEL == kDBN || MODEL == kCCM)) ? 5 : 1; };
+
 template <int MODEL, bool VARLEN, int PER>
-__global__ void __launch_bounds__(256) em_window_persistent_kernel(KArgs A, int n_win) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWaves<MODEL, VARLEN>::value)))
+em_window_persistent_kernel(KArgs A, int n_win) {
     constexpr int T = Shape<MODEL>::T;
     constexpr int K = Shape<MODEL>::K;
     constexpr int KACC = Shape<MODEL>::KACC;
