@@ -53,6 +53,7 @@ struct GArgs {
     float* grad[2];                // gradient output (GRAD mode)
     unsigned long long* gacc[2];   // fixed-point sums of split entries
     const int2* counts;
+    const uint16_t* counts16;     // packed counts (unpack_counts)
     float* sh_val;
     float* sh_num;
     float* sh_den;
@@ -384,13 +385,13 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
     constexpr int PER = NT >= 1024 ? 2 : 0;
     constexpr int PERA = PER > 0 ? PER : 1;
     const bool interior = WIN && (it.flags & kItemInterior) != 0;
-    int2 f_oc[PERA];
+    uint32_t f_oc[PERA];
     float f_logit[PERA][T], f_m[PERA][T], f_v[PERA][T];
 #pragma unroll
     for (int u = 0; u < PER; u++) {
         const int i = u * NT + tid;
         const bool on = interior && i < W;
-        f_oc[u] = on ? G.counts[it.base + i] : make_int2(0, 0);
+        f_oc[u] = on ? G.counts16[it.base + i] : 0u;
 #pragma unroll
         for (int t = 0; t < T; t++) {
             f_logit[u][t] = (ADAM && on) ? G.logit[t][it.base + i] : 0.f;
@@ -456,9 +457,10 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
             const int i = u * NT + tid;
             if (!interior || i >= W) continue;
             const int p = it.base + i;
+            const int2 oc = unpack_counts(f_oc[u], G.counts, p);
 #pragma unroll
             for (int t = 0; t < T; t++) {
-                const float g = entry_gradient<MODEL>(t, fix_to_double(lacc[i * T + t]), f_oc[u], lval[i * T + t],
+                const float g = entry_gradient<MODEL>(t, fix_to_double(lacc[i * T + t]), oc, lval[i * T + t],
                                                       ADAM && G.use_prior);
                 if (ADAM) {
                     G.val[t][p] = ad.apply(f_logit[u][t], f_m[u][t], f_v[u][t], g);
@@ -474,14 +476,14 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
         // loads issued before its first store.
         constexpr int U = 4;
         for (int i0 = tid + (interior ? PER * NT : 0); i0 < W; i0 += U * NT) {
-            int2 oc[U];
+            uint32_t oc[U];
             float lg[U][T], am[U][T], av[U][T];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int i = i0 + u * NT;
                 const bool on = interior && i < W;
                 const int p = it.base + (on ? i : 0);
-                oc[u] = on ? G.counts[p] : make_int2(0, 0);
+                oc[u] = on ? G.counts16[p] : 0u;
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     lg[u][t] = (ADAM && on) ? G.logit[t][p] : 0.f;
@@ -494,11 +496,12 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
                 const int i = i0 + u * NT;
                 if (i >= W) continue;
                 const int p = it.base + i;
+                const int2 ocu = interior ? unpack_counts(oc[u], G.counts, p) : make_int2(0, 0);
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     const unsigned long long v = lacc[i * T + t];
                     if (interior) {
-                        const float g = entry_gradient<MODEL>(t, fix_to_double(v), oc[u], lval[i * T + t],
+                        const float g = entry_gradient<MODEL>(t, fix_to_double(v), ocu, lval[i * T + t],
                                                               ADAM && G.use_prior);
                         if (ADAM) {
                             G.val[t][p] = ad.apply(lg[u][t], am[u][t], av[u][t], g);
@@ -637,6 +640,7 @@ GArgs make_gargs(mc_model* m, int batch) {
         G.gacc[t] = m->d_gacc[tt];
     }
     G.counts = m->d_counts;
+    G.counts16 = m->d_counts16;
     G.sh_val = m->d_sh_val[m->cur];
     G.sh_num = m->d_sh_num[m->cur];
     G.sh_den = m->d_sh_den[m->cur];
