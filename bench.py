@@ -268,12 +268,12 @@ def main():
     # What this engine's own layout moves per launch: 2 B per shown rank for
     # sessions of window items (16-bit window-relative indices), 4 B for the
     # others, the mask (and length), and per entry an item owns the value in
-    # and out of each table plus its 8-byte counts.
+    # and out of each table plus its 2-byte packed counts.
     si = cm.stream_info()
     n_w = si["n_window_sessions"]
     extra = 2 + (1 if batched else 0)
     layout_sess = n_w * (2 * mean_len + extra) + (n_train - n_w) * (4 * mean_len + extra)
-    layout_bytes = layout_sess + (8 * n_tab + 8) * (ds.n_qd - n_shared) if segmented else layout_sess
+    layout_bytes = layout_sess + (8 * n_tab + 2) * (ds.n_qd - n_shared) if segmented else layout_sess
 
     ev = cm.evaluate(ds.test)
     log(f"held-out LL {ev['log_likelihood']:.6f}, perplexity {ev['perplexity']:.6f}")
