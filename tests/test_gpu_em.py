@@ -237,6 +237,29 @@ def test_variable_length_lists_match_oracle(model, mode):
     np.testing.assert_array_equal(ev["rank_count"], oev["rank_count"])
 
 
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_duplicate_documents_in_a_result_list_match_oracle(model):
+    """A document shown at two ranks of one result list (real logs have
+    them): both ranks share one query-document entry, whose counts and sums
+    take both occurrences.  Parameters after 50 iterations within 1e-5."""
+    name, n, nq = CASES[model]
+    s = synth.generate(synth.SynthConfig(n_sessions=n, n_queries=nq, vocab=40 * nq, model=name, seed=300 + model))
+    rng = np.random.default_rng(7)
+    dup = rng.random(n) < 0.15
+    docs = s.docs.copy()
+    docs[dup, 6] = docs[dup, 1]
+    ds = oracle.from_sessions(s.query, docs, s.clicks, 0.2)
+    tr = ds["train"]
+    assert any(len(set(row)) < 10 for row in tr["pidx"][:2000])
+    om = oracle.OracleModel(model, tr["query"], tr["pidx"], tr["clicks"], ds["n_qd"], accum_fp64=True)
+    om.iterate(ITERS)
+    with run_gpu(model, ds) as cm:
+        for t in range(om.n_tables()):
+            oval = om.get_table(t)[2]
+            err = float(np.max(np.abs(cm.get_values(t) - oval)))
+            assert err < TOL_PARAM, f"table {t}: max |value - oracle| = {err}"
+
+
 @pytest.mark.parametrize("model", [PBM, DBN], ids=["PBM", "DBN"])
 def test_sessions_not_grouped_by_query_fall_back_to_atomics(model):
     """mc_init_parameters with the training sessions in random order (query
