@@ -311,10 +311,12 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 // blockDim.x == 256 and width <= 256 * PER.
 // Waves per SIMD the persistent kernel asks the compiler to fit: ten-rank DBN
 // and CCM go from 4 waves (108 and 114 VGPRs) to 5 (96 VGPRs, spilling
-// 5-8 dwords), which measured 1-3 % faster; PBM already fits 6 (7 measured
-// slower) and UBM's 55 accumulators leave no room.
+// 5-8 dwords), which measured 1-3 % faster; the uneven-length UBM kernel
+// goes from 2 waves (181 VGPRs) to 3 (168 VGPRs, 2 dwords spilled), 22 %
+// faster; PBM already fits 6 (7 measured slower) and ten-rank UBM's 55
+// accumulators leave no room.
 template <int MODEL, bool VARLEN>
-struct PersistWaves { static constexpr int value = (!VARLEN && (MODEL == kDBN || MODEL == kCCM)) ? 5 : 1; };
+struct PersistWaves { static constexpr int value = (!VARLEN && (MODEL == kDBN || MODEL == kCCM)) ? 5 : (VARLEN && MODEL == kUBM) ? 3 : 1; };
 
 template <int MODEL, bool VARLEN, int PER>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWaves<MODEL, VARLEN>::value)))
