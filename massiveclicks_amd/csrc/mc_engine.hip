@@ -313,8 +313,8 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 // and CCM go from 4 waves (108 and 114 VGPRs) to 5 (96 VGPRs, spilling
 // 5-8 dwords), which measured 1-3 % faster; the uneven-length UBM kernel
 // goes from 2 waves (181 VGPRs) to 3 (168 VGPRs, 2 dwords spilled), 22 %
-// faster; PBM already fits 6 (7 measured slower) and ten-rank UBM's 55
-// accumulators leave no room.
+// faster; PBM already fits 6 (7 measured slower), and ten-rank UBM at 5
+// waves (96 VGPRs, 40 bytes of spills) measured 17 % slower than at 4.
 template <int MODEL, bool VARLEN>
 struct PersistWaves { static constexpr int value = (!VARLEN && (MODEL == kDBN || MODEL == kCCM)) ? 5 : (VARLEN && MODEL == kUBM) ? 3 : 1; };
 
