@@ -8,7 +8,7 @@
 //   held-out LL and perplexity           evaluation.cpp:25-66
 //   parameter CSV files (-o)             Communicate::output_parameters
 // Multi-node and multi-GPU flags are accepted with their single-worker
-// meaning; CPU-only execution (-e 1) is not part of this build.
+// meaning; CPU-only execution (-e 1) is not implemented (the engine is GPU-only).
 
 #include <hip/hip_runtime_api.h>
 #include <unistd.h>
@@ -61,11 +61,11 @@ void show_help() {
         "  -m, --model-type <id>      0 PBM, 1 CCM, 2 DBN, 3 UBM (default 0)\n"
         "  -t, --test-share <f>       share of sessions held out (default 0.2)\n"
         "  -o, --output <prefix>      write <prefix>_<parameter>.csv files\n"
-        "  -g, --n-gpus <n>           GPUs to use (this build: 1)\n"
+        "  -g, --n-gpus <n>           GPUs to use (must be 1)\n"
         "  -n, --n-threads <n>        host threads (reported)\n"
         "  -p, --partition-type <id>  multi-GPU partitioning (no effect with one GPU)\n"
         "  -j, --job-id <id>          job id (reported)\n"
-        "  -e, --exec-mode <id>       0 GPU-only (this build)\n"
+        "  -e, --exec-mode <id>       0 GPU-only (the only mode)\n"
         "      --reduce-mode <id>     0 deterministic segmented (default), 1 float atomics\n"
         "      --autotune <0|1>       measure and pick the EM launch options first\n"
         "      --fit <em|adam>        plain EM (default) or Adam on the log-likelihood gradient;\n"
@@ -141,11 +141,11 @@ int main(int argc, char** argv) {
         return 1;
     }
     if (config.exec_mode != 0) {
-        std::cerr << "Error: only GPU-only execution (-e 0) is part of this build." << std::endl;
+        std::cerr << "Error: only GPU-only execution (-e 0) is supported." << std::endl;
         return 1;
     }
     if (config.max_gpus != 1) {
-        std::cerr << "Error: this build trains on one GPU (-g 1); the query-sharded multi-GPU design is not built."
+        std::cerr << "Error: training runs on one GPU (-g 1); query-sharded multi-GPU training is not implemented."
                   << std::endl;
         return 1;
     }

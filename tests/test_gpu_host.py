@@ -19,8 +19,8 @@ CLI = os.path.join(os.path.dirname(HERE), "massiveclicks_amd", "lib", "mclicks")
 
 def test_cli_trains_the_pbm_40k_log_like_the_oracle(tmp_path):
     """The 40,000-session PBM workload in the raw text format, run by the
-    driver as `-r log -s 40000 -i 50 -m 0 -t 0.2 -o prefix`: held-out LL and
-    per-rank perplexity match the oracle (1e-6 relative; the driver prints 8
+    command line as `-r log -s 40000 -i 50 -m 0 -t 0.2 -o prefix`: held-out LL and
+    per-rank perplexity match the oracle (1e-6 relative; the command line prints 8
     significant digits) and the attractiveness/examination CSV files hold the
     oracle's values within the 1e-5 parameter bar (plus print rounding)."""
     s = synth.generate(synth.config_pbm_40k())
