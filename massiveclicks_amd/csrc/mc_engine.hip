@@ -304,14 +304,15 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 // assignment keeps the per-lane float sums, and so the results, identical
 // from run to run.
 // While item k is processed, the window values of the workgroup's next item
-// and its first session are already in flight (registers), and the constants
-// of item k's entries arrive during its session loop, so the round trips of
-// staging and flushing overlap with the E-step.  The shared-table counts stay
+// and its first session are in flight (registers), and the constants of
+// item k's entries arrive during its session loop, so the round trips of
+// staging and flushing overlap with the E-step and the flush.  The shared-table counts stay
 // in registers across items and are reduced once per workgroup.  Requires
 // blockDim.x == 256 and width <= 256 * PER.
 // Waves per SIMD the persistent kernel asks the compiler to fit: ten-rank DBN
-// and CCM go from 4 waves (108 and 114 VGPRs) to 5 (96 VGPRs, spilling
-// 5-8 dwords), which measured 1-3 % faster; the uneven-length UBM kernel
+// and CCM go from 4 waves (108 and 114 VGPRs) to 5 (96 VGPRs; with the
+// window loaded after the session loop, DBN spills nothing and CCM three
+// dwords), which measured 1-3 % faster; the uneven-length UBM kernel
 // goes from 2 waves (181 VGPRs) to 3 (168 VGPRs, 2 dwords spilled), 22 %
 // faster; PBM already fits 6 (7 measured slower), and ten-rank UBM at 5
 // waves (96 VGPRs, 40 bytes of spills) measured 17 % slower than at 4.
@@ -378,18 +379,33 @@ em_window_persistent_kernel(KArgs A, int n_win) {
         const Item itn = nxt < n_win ? A.items[nxt] : Item{0, 0, 0, 0, 0, 0};
         // In flight during the session loop: next window values, this
         // window's counts.
+        // Ten-rank kernels load the next window after the session loop
+        // instead (its latency then overlaps the flush), which frees the
+        // window registers during the loop: fewer spills at 5 waves per SIMD,
+        // 1-3 % faster; the uneven-length kernels measured 1-3 % slower so.
+        constexpr bool LATE_WINDOW = !VARLEN;
         uint32_t cnt[PER];
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int i = u * NT + tid;
+            if (!LATE_WINDOW) {
 #pragma unroll
-            for (int t = 0; t < T; t++) wv[u][t] = (i < itn.width) ? A.tab[t].val_in[itn.base + i] : 0.f;
+                for (int t = 0; t < T; t++) wv[u][t] = (i < itn.width) ? A.tab[t].val_in[itn.base + i] : 0.f;
+            }
             cnt[u] = (i < W) ? A.counts16[it.base + i] : 0u;
         }
         {
             EmitLds<T> em{lacc};
             const SrcLds<T> lsrc{lval};
             run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+        }
+        if (LATE_WINDOW) {
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int i = u * NT + tid;
+#pragma unroll
+                for (int t = 0; t < T; t++) wv[u][t] = (i < itn.width) ? A.tab[t].val_in[itn.base + i] : 0.f;
+            }
         }
         // First session of the next item goes out before the flush.
         load_session<VARLEN, true>(st, itn.s0 + tid, itn.s1, m0, L0, pv0);
