@@ -1835,11 +1835,16 @@ mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, s
     MC_REQUIRE(bytes && model_type >= 0 && model_type <= 3 && n >= 0 && n_qd >= 0, "invalid argument");
     int T, K, KACC;
     shape_of(model_type, &T, &K, &KACC);
-    // sessions: at most 40 B pidx (20 B for sessions of window items) + 2 B
-    // mask; per table entry: 2 values, 2 numerators,
-    // (base, den) and an 8-byte accumulator; shared partials per ~1k sessions.
-    size_t b = (size_t)n * 42 + (size_t)n_qd * T * (16 + 8 + 8) + (size_t)K * 32;
-    b += (size_t)KACC * 8 * (size_t)(n / 1024 + 1) + (size_t)(n / 1024 + 1) * sizeof(Item);
+    // EM buffers, for the default layout.  Sessions: at most 40 B of indices
+    // (20 B for sessions of window items), 2 B mask, 1 B length.  Per entry
+    // and table: two values, two numerators, an 8-byte accumulator; per
+    // entry: 8 B counts, 2 B packed counts, 4 B if merged across items.
+    // Work items of at least 64 sessions (24 B each, 8 B of batch lists).
+    const size_t items = (size_t)(n / 64 + 1);
+    // Shared tables: values, numerators, denominators (two each), FP64 bases,
+    // accumulator map; plus the 16-byte minimum of small allocations.
+    size_t b = (size_t)n * 43 + (size_t)n_qd * ((size_t)T * 24 + 14) + (size_t)K * 64;
+    b += (size_t)KACC * 8 + items * (sizeof(Item) + 8) + 256;
     *bytes = b;
     return MC_OK;
 }

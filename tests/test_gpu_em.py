@@ -109,6 +109,18 @@ def test_layout_options_keep_parity(model, opts):
             assert err < TOL_PARAM, f"{opts} {lay} {si} table {t}: {err}"
 
 
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_memory_footprint_bounds_the_usage(model):
+    """compute_memory_footprint (base.cuh:32) before a run bounds what the
+    model allocates (get_memory_usage) with the default layout, within 2x."""
+    ds, _, _ = workload(model)
+    n = len(ds["train"]["query"])
+    with run_gpu(model, ds, iters=1) as cm:
+        used = cm.get_memory_usage()
+        est = ClickModel.compute_memory_footprint(model, n, ds["n_qd"])
+        assert used <= est <= 2 * used, (used, est)
+
+
 def test_window_larger_than_lds_is_rejected():
     """Windows of 16384 entries (12 bytes each for one table) do not fit a
     workgroup's LDS: init reports it instead of failing at launch."""
