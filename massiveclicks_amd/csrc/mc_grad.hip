@@ -36,7 +36,12 @@ namespace mc {
 namespace {
 
 constexpr float kLogitMax = 13.8f;   // sigmoid(13.8) = 1 - 1.0e-6, the Param::value clamp
-constexpr int kShAccStride = 64;     // shared-table sums of one step (>= 55 accumulators)
+constexpr int kShAccStride = 64;     // shared-table entries (>= 55 accumulators)
+// A step's shared-table sums are split over kShParts copies, one per XCD
+// (consecutive workgroups run on consecutive XCDs), so that the items'
+// atomics do not all queue on one address; readers add the copies.
+constexpr int kShParts = 8;
+constexpr int kShAccSlot = kShParts * kShAccStride;
 
 struct GArgs {
     const uint16_t* pidx16;
@@ -276,6 +281,13 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 
 // Merge work of grad_merge_kernel.
 __device__ __forceinline__ unsigned long long load_sum(const unsigned long long* p) { return *p; }
+// The sum of the kShParts copies of shared-table entry p[0].
+__device__ __forceinline__ unsigned long long shared_sum(const unsigned long long* p) {
+    unsigned long long s = 0;
+#pragma unroll
+    for (int q = 0; q < kShParts; q++) s += p[q * kShAccStride];
+    return s;
+}
 
 // Adam step of shared-table entry k from the sum of a step of batch
 // `batch` (scaled by `scale`); returns the new value and, when `write`,
@@ -302,7 +314,7 @@ __device__ __forceinline__ float shared_adam(const GArgs& G, int k, double sum, 
 
 template <int MODEL, bool ADAM>
 __device__ __forceinline__ void merge_shared_entry(const GArgs& G, int k) {
-    const double sum = fix_to_double(load_sum(&G.sh_acc[k]));
+    const double sum = fix_to_double(shared_sum(&G.sh_acc[k]));
     if (ADAM) shared_adam(G, k, sum, G.sh_scale, G.batch, true);
     else G.sh_grad[k] = (float)sum;
 }
@@ -361,9 +373,9 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
         // shared tables' new values for its own sessions (workgroup 0 stores
         // them), and the workgroups share the previous batch's split entries.
         for (int k = tid; k < K; k += NT)
-            sh[k] = shared_adam(G, k, fix_to_double(G.prev_acc[k]), G.prev_scale, G.prev_batch, blockIdx.x == 0);
+            sh[k] = shared_adam(G, k, fix_to_double(shared_sum(&G.prev_acc[k])), G.prev_scale, G.prev_batch, blockIdx.x == 0);
         if (blockIdx.x == 0)
-            for (int j = tid; j < G.kacc; j += NT) G.zero_acc[j] = 0ull;
+            for (int j = tid; j < kShAccSlot; j += NT) G.zero_acc[j] = 0ull;
         const AdamStep pad = adam_step(G, G.steps[0] + 1);
         for (int64_t i = (int64_t)blockIdx.x * NT + tid; i < G.prev_cnt; i += (int64_t)gridDim.x * NT)
             merge_split_entry<MODEL, true>(G, G.prev_off, i, pad);
@@ -440,7 +452,7 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
         long long v = 0;
 #pragma unroll
         for (int w = 0; w < NW; w++) v += red[k][w];
-        if (v != 0) atomicAdd(&G.sh_acc[k], (unsigned long long)v);
+        if (v != 0) atomicAdd(&G.sh_acc[(blockIdx.x % kShParts) * kShAccStride + k], (unsigned long long)v);
     }
     if (!ADAM && tid == 0) {
         double lsum = 0.0;
@@ -529,9 +541,9 @@ __global__ void __launch_bounds__(256) grad_merge_kernel(GArgs G) {
     if (blockIdx.x == 0) {
         if (tid < G.K) merge_shared_entry<MODEL, ADAM>(G, tid);
         __syncthreads();
-        for (int j = tid; j < G.kacc; j += blockDim.x) G.sh_acc[j] = 0ull;
+        for (int j = tid; j < kShAccSlot; j += blockDim.x) G.sh_acc[j] = 0ull;
         if (G.zero_acc)
-            for (int j = tid; j < 3 * kShAccStride; j += blockDim.x) G.zero_acc[j] = 0ull;
+            for (int j = tid; j < 3 * kShAccSlot; j += blockDim.x) G.zero_acc[j] = 0ull;
     } else {
         const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
         AdamStep ad;
@@ -594,8 +606,8 @@ mc_status ensure_grad_buffers(mc_model* m) {
     MC_TRY(dalloc(m, &m->d_sh_am, (size_t)K * 4));
     MC_TRY(dalloc(m, &m->d_sh_av, (size_t)K * 4));
     MC_TRY(dalloc(m, &m->d_sh2, (size_t)4 * kShAccStride * 4));     // alternate logit, m, v, value
-    MC_TRY(dalloc(m, &m->d_gsh, (size_t)3 * kShAccStride * 8));     // three steps of shared-table sums
-    MC_HIP(hipMemset(m->d_gsh, 0, (size_t)3 * kShAccStride * 8));
+    MC_TRY(dalloc(m, &m->d_gsh, (size_t)3 * kShAccSlot * 8));       // three steps of shared-table sums
+    MC_HIP(hipMemset(m->d_gsh, 0, (size_t)3 * kShAccSlot * 8));
     MC_TRY(dalloc(m, &m->d_sh_grad, (size_t)K * 4));
     MC_TRY(dalloc(m, &m->d_sh_prior, (size_t)K * 8));
     MC_TRY(dalloc(m, &m->d_ll_part, (size_t)m->n_items * 8));
@@ -735,15 +747,15 @@ void launch_fused_step_t(mc_model* m, int b) {
         G.wr_av = wr[2];
         G.wr_val = wr[3];
         G.wr_main = wr == A4;
-        G.sh_acc = m->d_gsh + (size_t)(b % 3) * kShAccStride;
+        G.sh_acc = m->d_gsh + (size_t)(b % 3) * kShAccSlot;
         if (b > 0) {
             G.prev_batch = b - 1;
             G.prev_off = m->shared_off[(size_t)b - 1];
             G.prev_cnt = m->shared_cnt[(size_t)b - 1];
             const int64_t nb = m->batch_sessions[(size_t)b - 1];
             G.prev_scale = nb > 0 ? (double)m->N / (double)nb : 1.0;
-            G.prev_acc = m->d_gsh + (size_t)((b - 1) % 3) * kShAccStride;
-            G.zero_acc = m->d_gsh + (size_t)((b + 1) % 3) * kShAccStride;
+            G.prev_acc = m->d_gsh + (size_t)((b - 1) % 3) * kShAccSlot;
+            G.zero_acc = m->d_gsh + (size_t)((b + 1) % 3) * kShAccSlot;
         }
         const size_t bi = (size_t)b;
         const int64_t nw = m->win_cnt[bi], ng = m->glob_cnt[bi];
@@ -773,7 +785,7 @@ void launch_fused_step_t(mc_model* m, int b) {
     G.sh_am = rd[1];
     G.sh_av = rd[2];
     G.sh_val = rd[3];
-    G.sh_acc = m->d_gsh + (size_t)((B - 1) % 3) * kShAccStride;
+    G.sh_acc = m->d_gsh + (size_t)((B - 1) % 3) * kShAccSlot;
     G.zero_acc = m->d_gsh;
     const unsigned grid = (unsigned)(1 + (G.shared_cnt + 255) / 256);
     hipLaunchKernelGGL((grad_merge_kernel<MODEL, true>), dim3(grid), dim3(256), 0, m->stream, G);
