@@ -473,15 +473,22 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
             ta[5] += n3 + phi6 * inv;
             ex = t2 * (1 - a) + t3 * a;
         } else {
+            // The factor sum d1 and the examination update's denominator d2
+            // share one reciprocal: q = 1/(d1 d2), 1/d1 = d2 q, 1/d2 = d1 q.
+            // Both are at least 1e-6 (clamped parameters), so d1 d2 is a
+            // normal float.
             const float E = ex;
             const float phi0 = 1 - E;
             const float phi4 = (1 - t1) * E;
             const float phi5 = t1 * Tn * E;
-            const float inv = __builtin_amdgcn_rcpf(phi0 + phi4 + phi5);
+            const float d1 = phi0 + phi4 + phi5;
+            const float d2 = 1 - a * E;
+            const float q = __builtin_amdgcn_rcpf(d1 * d2);
+            const float inv = d2 * q;
             const float n1 = phi5 * inv;
             ta[0] += n1;
             ta[1] += n1 + phi4 * inv;
-            ex = ex * fdiv(t1 * (1 - a), 1 - a * ex);
+            ex = E * (t1 * (1 - a)) * (d1 * q);
         }
     }
 }
