@@ -126,6 +126,14 @@ __device__ __forceinline__ long long to_fix(float x) {
     return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
+// to_fix for a posterior x in [0, 1]: every float below 1 is at most
+// 1 - 2^-24, so x * 2^32 <= 2^32 - 256 converts exactly and the result equals
+// to_fix(x); the bound only changes x = 1 (by 256 * 2^-32).  Three
+// instructions instead of five.
+__device__ __forceinline__ unsigned long long to_fix_unit(float x) {
+    return (unsigned long long)(uint32_t)fminf(x * kFixScale, 4294967040.0f);
+}
+
 // a / b through the hardware reciprocal (1 ulp); used in the DBN and CCM
 // chains, where the reference's own arithmetic (log/exp of products) differs
 // from these restructured products by more than this anyway.
@@ -139,12 +147,17 @@ __device__ __forceinline__ int last_click(uint32_t m) {
 // --------------------------------------------------------------------------
 // Emitters: where a per-qd expected count goes.
 // --------------------------------------------------------------------------
-// add(t, r, p, x): expected count x of per-qd table t for the entry p shown
-// at rank r of the current session.
+// add(t, r, p, x): expected count x, a posterior in [0, 1], of per-qd table
+// t for the entry p shown at rank r of the current session; add_wide takes
+// any non-negative x (the CCM's last-click attractiveness term can exceed 1;
+// the last-click terms, once per session, always take it).
 template <int T>
 struct EmitLds {            // fixed-point LDS window, relative indices
     unsigned long long* acc;   // [width][T]
     __device__ __forceinline__ void add(int t, int, int p, float x) {
+        atomicAdd(&acc[p * T + t], to_fix_unit(x));
+    }
+    __device__ __forceinline__ void add_wide(int t, int, int p, float x) {
         atomicAdd(&acc[p * T + t], (unsigned long long)to_fix(x));
     }
 };
@@ -152,6 +165,9 @@ struct EmitGlobalFix {      // fixed-point global (item without a window)
     unsigned long long* g0;
     unsigned long long* g1;
     __device__ __forceinline__ void add(int t, int, int p, float x) {
+        atomicAdd(t == 0 ? &g0[p] : &g1[p], to_fix_unit(x));
+    }
+    __device__ __forceinline__ void add_wide(int t, int, int p, float x) {
         atomicAdd(t == 0 ? &g0[p] : &g1[p], (unsigned long long)to_fix(x));
     }
 };
@@ -159,6 +175,7 @@ template <int T>
 struct EmitLdsFloat {       // float LDS window (float-atomic mode), relative indices
     float* acc;                // [width][T]
     __device__ __forceinline__ void add(int t, int, int p, float x) { atomicAdd(&acc[p * T + t], x); }
+    __device__ __forceinline__ void add_wide(int t, int r, int p, float x) { add(t, r, p, x); }
 };
 struct EmitGlobalFloat {    // float atomics (reference-like mode)
     float* g0;
@@ -166,6 +183,7 @@ struct EmitGlobalFloat {    // float atomics (reference-like mode)
     __device__ __forceinline__ void add(int t, int, int p, float x) {
         unsafeAtomicAdd(t == 0 ? &g0[p] : &g1[p], x);
     }
+    __device__ __forceinline__ void add_wide(int t, int r, int p, float x) { add(t, r, p, x); }
 };
 // Sessions whose entries no other session touches (a query group of one
 // session): the counts stay in registers, indexed by rank, and the lane
@@ -179,6 +197,7 @@ struct EmitRegs {
             for (int r = 0; r < kL; r++) x[t][r] = 0.f;
     }
     __device__ __forceinline__ void add(int t, int r, int, float v) { x[t][r] += v; }
+    __device__ __forceinline__ void add_wide(int t, int r, int p, float v) { add(t, r, p, v); }
 };
 
 // --------------------------------------------------------------------------
@@ -312,6 +331,8 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
     const int lcr = last_click(mm);
     // Pass 1, ranks 0..L-1: exam[r] (compute_exm_car) and the no-click tail
     // products V_k; only exam[r] for r > lcr and V_k for k < L - lcr are used.
+    // The examination factor t + g a (1 - s) equals g (1 - a s): with
+    // ga = g a, one multiply-add gives it and one subtraction gives t.
     float exam[kL], V[kL];
     {
         float e = 1.f, v = 1.f, G = 1.f;
@@ -321,10 +342,10 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
             V[r] = v;
             if (!VARLEN || r < L) {
                 const float2 as = src.pair(pv[r]);
-                const float t = g * (1 - as.x);
-                e = e * (t + g * as.x * (1 - as.y));
+                const float ga = g * as.x;
+                e = e * (g - ga * as.y);
                 v = v - as.x * G;
-                G = G * t;
+                G = G * (g - ga);
             }
         }
     }
@@ -358,7 +379,7 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
             else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
             if (r == lcr) {
                 const float sv = src.get(1, pv[r]);
-                em.add(1, r, pv[r], fdiv(sv, 1 - (1 - sv) * g * car));
+                em.add_wide(1, r, pv[r], fdiv(sv, 1 - (1 - sv) * g * car));
                 const float phi4 = (1 - sv) * (1 - g);
                 const float phi5 = (1 - sv) * g * Tn;
                 const float inv = __builtin_amdgcn_rcpf(phi4 + phi5 + sv);
@@ -459,7 +480,7 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
         if (VARLEN) Tn = (r + 1 < L) ? pick10(V, L - 1 - r) : 1.f;
         else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
         if (r == lcr) {
-            em.add(0, r, pv[r], fdiv(a, 1 - (t2 * (1 - a) + t3 * a) * car));
+            em.add_wide(0, r, pv[r], fdiv(a, 1 - (t2 * (1 - a) + t3 * a) * car));
             const float phi4 = (1 - a) * (1 - t2);
             const float phi5 = (1 - a) * t2 * Tn;
             const float phi6 = a * (1 - t3);
