@@ -364,13 +364,14 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
         }
     }
     // Pass 3, ranks lcr..L-1: satisfaction at the last click, gamma factors.
-    // After the last click the conditional examination probability is kept
-    // as a ratio E = u / v: E_{r+1} = E_r g(1-a_r) / (1 - a_r E_r) becomes
-    // u' = u g(1-a_r), v' = v - a_r u, and the factor ratios of an unclicked
-    // rank, phi5/sum = g T E / (1 - g E (1-T)) and (phi4+phi5)/sum =
-    // E (1-g+gT) / (1 - g E (1-T)), share one reciprocal of v - g u (1-T).
+    // After the last click the conditional examination probability E
+    // follows E_{r+1} = E_r g(1-a_r) / d2, d2 = 1 - a_r E_r, and the factor
+    // ratios of an unclicked rank are phi5/sum = g T E / d1 and
+    // (phi4+phi5)/sum = E (1-g+gT) / d1, d1 = 1 - g E (1-T).  Both divisions
+    // share one reciprocal q = 1/(d1 d2): 1/d1 = d2 q, 1/d2 = d1 q.  E <= g,
+    // so d1, d2 >= 1 - g >= 1e-6 (clamped gamma) and d1 d2 is a normal float.
     __asm__ volatile("" ::: "memory");
-    float u = 0.f, v = 1.f;
+    float E = 0.f;
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         if ((!VARLEN || r < L) && r >= lcr) {
@@ -385,18 +386,16 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
                 const float inv = __builtin_amdgcn_rcpf(phi4 + phi5 + sv);
                 gn += phi5 * inv;
                 gd += (phi4 + phi5) * inv;
-                u = g * (1 - sv);
-                v = 1.f;
+                E = g * (1 - sv);
             } else {
                 const float a = src.get(0, pv[r]);
-                const float inv = __builtin_amdgcn_rcpf(v - g * u * (1 - Tn));
-                gn += g * Tn * u * inv;
-                gd += (1 - g + g * Tn) * u * inv;
-                const float u2 = u * (g * (1 - a));
-                v = v - a * u;
-                u = u2;
-                // Keep the ratio representable after many small factors.
-                if (v < 0x1p-60f) { u *= 0x1p60f; v *= 0x1p60f; }
+                const float d1 = 1 - g * E * (1 - Tn);
+                const float d2 = 1 - a * E;
+                const float q = __builtin_amdgcn_rcpf(d1 * d2);
+                const float w = E * (d2 * q);
+                gn += g * Tn * w;
+                gd += (1 - g + g * Tn) * w;
+                E = E * (g * (1 - a)) * (d1 * q);
             }
         }
     }
