@@ -424,8 +424,11 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
     const int lcr = last_click(mm);
     // Pass 1, ranks 0..L-1: exam[r] (compute_exm_car) and the tail products
     // V_k of the fresh-list chain (G_{k+1} = G_k tau1 (1 - a_k)).
+    // The examination factor (1-a) tau1 + a((1-a) tau2 + a tau3) is the
+    // quadratic tau1 + a (tau2 - tau1) + a^2 (tau3 - tau2): two multiply-adds.
     float exam[kL], V[kL];
     {
+        const float c1 = t2 - t1, c2 = t3 - t2;
         float e = 1.f, v = 1.f, G = 1.f;
 #pragma unroll
         for (int r = 0; r < kL; r++) {
@@ -433,10 +436,9 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
             V[r] = v;
             if (!VARLEN || r < L) {
                 const float a = src.get(0, pv[r]);
-                const float t = (1 - a) * t1;
-                e = e * (t + a * ((1 - a) * t2 + a * t3));
+                e = e * ((c2 * a + c1) * a + t1);
                 v = v - a * G;
-                G = G * t;
+                G = G * (t1 - a * t1);
             }
         }
     }
