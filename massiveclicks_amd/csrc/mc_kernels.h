@@ -272,6 +272,26 @@ __device__ __forceinline__ void ubm_slots(uint32_t mm, int (&slot)[kL]) {
 template <class Src, class Emit>
 __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             const float* sh, float (&acc)[55]) {
+    if (mm == 0) {
+        // No click: every rank's slot is its row's "no previous click" one,
+        // known at compile time, so the select over the row is not needed.
+        // Sessions are sorted by last click inside an item, so click-free
+        // sessions fill whole wavefronts.
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                const int row = r * (r + 1) / 2;
+                const int p = pv[r];
+                const float atr = src.get(0, p);
+                const float ex = sh[row + kL - 1];
+                const float ae = atr * ex;
+                const float inv = __builtin_amdgcn_rcpf(1 - ae);
+                em.add(0, r, p, (atr - ae) * inv);
+                acc[row + kL - 1] += (ex - ae) * inv;
+            }
+        }
+        return;
+    }
     int slot[kL];
     ubm_slots(mm, slot);
 #pragma unroll
