@@ -190,6 +190,41 @@ def test_segmented_mode_is_deterministic_and_graph_equals_eager(model):
             np.testing.assert_array_equal(d0, d1)
 
 
+@pytest.mark.parametrize("uneven", [False, True], ids=["ten", "uneven"])
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_values_at_the_extremes_stay_finite_and_in_range(model, uneven):
+    """Every per-qd value at 1e-6 or at the clamp 1 - 1e-6, every shared
+    value at the clamp: the chains' denominators (1 - a E and 1 - g E (1 - T)
+    after the last click, 1 - a e for PBM/UBM) are then as small as clamped
+    parameters allow, about 1e-6.  Ten iterations must keep every value
+    finite and in [0, 1 - 1e-6] and every numerator and denominator finite
+    and positive.  (Values that extreme leave no oracle bar to compare with:
+    both sides lose most digits of such differences.)"""
+    name = {PBM: "pbm", UBM: "ubm", DBN: "dbn", CCM: "ccm"}[model]
+    s = synth.generate(synth.SynthConfig(n_sessions=4000, n_queries=400, vocab=16000, model=name,
+                                         variable_length=uneven, seed=21))
+    ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2, length=s.length if uneven else None)
+    rng = np.random.default_rng(21)
+    clamp = np.float32(1.0) - np.float32(1e-6)
+    print(f"case: extremes model={model} uneven={uneven}", flush=True)
+    with ClickModel(model, device=0) as cm:
+        cm.init_from_dataset(ds)
+        nt = cm.n_tables()
+        for t, (_, _, size) in enumerate(cm.get_parameter_information()):
+            if t == nt - 1:
+                num = np.ones(size, np.float32)
+            else:
+                num = np.where(rng.random(size) < 0.5, np.float32(1e-6), np.float32(1.0)).astype(np.float32)
+            cm.set_parameters(t, num, np.ones(size, np.float32))
+        cm.em(10)
+        for t in range(nt):
+            num, den = cm.get_parameters(t)
+            assert np.all(np.isfinite(num)) and np.all(np.isfinite(den)), f"table {t}: non-finite sums"
+            assert np.all(num >= 0) and np.all(den > 0), f"table {t}: negative sums"
+            val = cm.get_values(t)
+            assert np.all((val >= 0) & (val <= clamp)), f"table {t}: value outside [0, 1 - 1e-6]"
+
+
 @pytest.mark.parametrize("model", [PBM, DBN], ids=["PBM", "DBN"])
 def test_stepwise_api_equals_em_iterations(model):
     ds, _, _ = workload(model)
