@@ -310,12 +310,12 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 // in registers across items and are reduced once per workgroup.  Requires
 // blockDim.x == 256 and width <= 256 * PER.
 // Waves per SIMD the persistent kernel asks the compiler to fit: ten-rank DBN
-// and CCM go from 4 waves (108 and 114 VGPRs) to 5 (96 VGPRs; with the
-// window loaded after the session loop, DBN spills nothing and CCM three
-// dwords), which measured 1-3 % faster; the uneven-length UBM kernel
-// goes from 2 waves (181 VGPRs) to 3 (168 VGPRs, 2 dwords spilled), 22 %
-// faster; PBM already fits 6 (7 measured slower), and ten-rank UBM at 5
-// waves (96 VGPRs, 40 bytes of spills) measured 17 % slower than at 4.
+// and CCM go from 4 waves (108 and 114 VGPRs) to 5 (95 and 94 VGPRs, no
+// spills), which measured 1-3 % faster; the uneven-length UBM kernel goes
+// from 2 waves (181 VGPRs) to 3 (168 VGPRs, 4 dwords spilled), 22 % faster.
+// Ten-rank PBM fits 7 without a hint (the host caps it at 6 per CU: 7
+// measured slower), and ten-rank UBM at 5 waves (96 VGPRs, 40 bytes of
+// spills) measured 17 % slower than at 4.
 template <int MODEL, bool VARLEN>
 struct PersistWaves { static constexpr int value = (!VARLEN && (MODEL == kDBN || MODEL == kCCM)) ? 5 : (VARLEN && MODEL == kUBM) ? 3 : 1; };
 
@@ -1429,8 +1429,8 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
         MC_HIP(e1);
         MC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, m->device));
         // At most 6 workgroups per CU: the ten-rank PBM kernel (71 VGPRs)
-        // fits 7, which measured 4 % slower (0.080 -> 0.084 ms per DBN-10M-
-        // sized PBM iteration) with one more window staged per CU.
+        // fits 7, which measured 4 % slower (0.080 -> 0.084 ms per iteration
+        // over 8M sessions) with one more window staged per CU.
         per_cu = std::min(per_cu, 6);
         m->persist_grid = std::max(1, std::min(m->n_win_items, per_cu * n_cu));
     }
