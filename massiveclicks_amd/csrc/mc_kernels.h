@@ -379,8 +379,8 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
     for (int r = kL - 1; r >= 0; r--) {
         if ((!VARLEN || r < L) && r > lcr) {
             const float a = src.get(0, pv[r]);
-            car = a + g * (1 - a) * car;
-            em.add(0, r, pv[r], fdiv(a * (1 - exam[r]), 1 - exam[r] * car));
+            car = a + (g - g * a) * car;
+            em.add(0, r, pv[r], fdiv(a - exam[r] * a, 1 - exam[r] * car));
         }
     }
     // Pass 3, ranks lcr..L-1: satisfaction at the last click, gamma factors.
@@ -471,8 +471,8 @@ __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t
     for (int r = kL - 1; r >= 0; r--) {
         if ((!VARLEN || r < L) && r > lcr) {
             const float a = src.get(0, pv[r]);
-            car = a + (1 - a) * t1 * car;
-            em.add(0, r, pv[r], fdiv((1 - exam[r]) * a, 1 - exam[r] * car));
+            car = a + (t1 - a * t1) * car;
+            em.add(0, r, pv[r], fdiv(a - exam[r] * a, 1 - exam[r] * car));
         }
     }
     // Pass 3, ranks 0..L-1: tau2/tau3 posteriors of earlier clicks, the last
