@@ -89,6 +89,25 @@ class Dataset:
             self._h = None
 
 
+def _sessions_arrays(sessions: SessionSet, what: str):
+    """Contiguous (pidx [n, 10] int32, clicks [n, 10] uint8, length [n] uint8
+    or None) of a session set, with the shapes the C ABI reads checked
+    first: the native side trusts the element counts it is given."""
+    n = len(sessions)
+    p = np.ascontiguousarray(sessions.pidx, np.int32)
+    c = np.ascontiguousarray(sessions.clicks, np.uint8)
+    if p.shape != (n, MAX_SERP):
+        raise ValueError(f"{what}: pidx must have shape ({n}, {MAX_SERP}), got {p.shape}")
+    if c.shape != (n, MAX_SERP):
+        raise ValueError(f"{what}: clicks must have shape ({n}, {MAX_SERP}), got {c.shape}")
+    ln = None
+    if sessions.length is not None:
+        ln = np.ascontiguousarray(sessions.length, np.uint8)
+        if ln.shape != (n,):
+            raise ValueError(f"{what}: length must have shape ({n},), got {ln.shape}")
+    return p, c, ln
+
+
 class ClickModel:
     """One click model on one GPU (ClickModel_Hst, base.cuh:30-53)."""
 
@@ -131,9 +150,9 @@ class ClickModel:
     # -- init_parameters (common.cu:31-61) --------------------------------------
     def init_parameters(self, train: SessionSet, n_qd: int):
         q = np.ascontiguousarray(train.query, np.int32)
-        p = np.ascontiguousarray(train.pidx, np.int32)
-        c = np.ascontiguousarray(train.clicks, np.uint8)
-        ln = None if train.length is None else np.ascontiguousarray(train.length, np.uint8)
+        if q.ndim != 1:
+            raise ValueError(f"init_parameters: query must be one-dimensional, got shape {q.shape}")
+        p, c, ln = _sessions_arrays(train, "init_parameters")
         check(_lib.lib().mc_init_parameters(self._h, len(q), ptr(q, i32p), ptr(p, i32p), ptr(c, u8p),
                                             ptr(ln, u8p), int(n_qd)))
         self.n_qd = int(n_qd)
@@ -198,6 +217,9 @@ class ClickModel:
 
     def set_logits(self, t: int, logits):
         x = np.ascontiguousarray(logits, np.float32)
+        size = self.get_parameter_information()[int(t)][2]
+        if x.shape != (size,):
+            raise ValueError(f"set_logits: table {t} takes logits of shape ({size},), got {x.shape}")
         check(_lib.lib().mc_set_logits(self._h, int(t), ptr(x, f32p)))
 
     # -- tables -----------------------------------------------------------------
@@ -230,16 +252,21 @@ class ClickModel:
         return np.where(v < np.float32(1.0) - np.float32(1e-6), v, np.float32(1.0) - np.float32(1e-6)).astype(np.float32)
 
     def set_parameters(self, t: int, num, den):
+        info = self.get_parameter_information()
+        if not 0 <= int(t) < len(info):
+            raise ValueError(f"set_parameters: no table {t}")
+        size = info[int(t)][2]
         num = np.ascontiguousarray(num, np.float32)
         den = np.ascontiguousarray(den, np.float32)
+        if num.shape != (size,) or den.shape != (size,):
+            raise ValueError(f"set_parameters: table {t} takes num and den of shape ({size},), "
+                             f"got {num.shape} and {den.shape}")
         check(_lib.lib().mc_set_parameters(self._h, int(t), ptr(num, f32p), ptr(den, f32p)))
 
     # -- evaluation (evaluation.cpp:25-66) --------------------------------------
     def evaluate(self, test: SessionSet, per_session: bool = False):
         n = len(test)
-        p = np.ascontiguousarray(test.pidx, np.int32)
-        c = np.ascontiguousarray(test.clicks, np.uint8)
-        ln = None if test.length is None else np.ascontiguousarray(test.length, np.uint8)
+        p, c, ln = _sessions_arrays(test, "evaluate")
         logp = np.empty((n, MAX_SERP), np.float32) if per_session else None
         full = np.empty((n, MAX_SERP), np.float32) if per_session else None
         llh = np.empty(n, np.float32) if per_session else None

@@ -19,7 +19,10 @@ import numpy as np
 
 
 def _assignments(n_bits):
-    return np.array(list(itertools.product((0, 1), repeat=n_bits)), dtype=np.int8)
+    """Every 0/1 vector of n_bits entries, in itertools.product order (the
+    last entry varies fastest)."""
+    i = np.arange(1 << n_bits, dtype=np.int64)[:, None]
+    return ((i >> (n_bits - 1 - np.arange(n_bits))[None, :]) & 1).astype(np.int8)
 
 
 def cascade_joint(model, a, s=None, g=None, taus=None):
@@ -194,3 +197,55 @@ def fisher_terms(model, clicks, a, s=None, g=None, taus=None):
         d_sh[1] += (w * np.where(clicked & (S[:, r] == 0), G[:, r] - t2, 0)).sum()
         d_sh[2] += (w * np.where(clicked & (S[:, r] == 1), G[:, r] - t3, 0)).sum()
     return d_attr, np.zeros(L), d_sh
+
+
+def independent_joint(a, e):
+    """PBM: every assignment of (A_r, E_r) for L ranks with A_r ~ Bern(a_r),
+    E_r ~ Bern(e_r) all independent.  Returns (clicks [M, L], prob [M])."""
+    L = len(a)
+    H = _assignments(2 * L)
+    A, E = H[:, :L], H[:, L:]
+    p = np.prod(np.where(A == 1, a[None, :], 1 - a[None, :]), axis=1)
+    p = p * np.prod(np.where(E == 1, e[None, :], 1 - e[None, :]), axis=1)
+    return (A * E).astype(np.int8), p
+
+
+def ubm_joint(a, exam_of):
+    """UBM: every assignment of (A_r, E_r) for L ranks generated rank by
+    rank, A_r ~ Bern(a_r) and E_r ~ Bern(exam_of(r, prev)) where prev is the
+    last clicked rank before r (-1 = none) of that assignment's own clicks.
+    Returns (clicks [M, L], prob [M])."""
+    L = len(a)
+    H = _assignments(2 * L)
+    A, E = H[:, :L], H[:, L:]
+    M = H.shape[0]
+    C = np.zeros((M, L), np.int8)
+    p = np.ones(M)
+    prev = np.full(M, -1)
+    for r in range(L):
+        e = np.array([exam_of(r, int(q)) for q in range(-1, r)])[prev + 1]
+        p *= np.where(A[:, r] == 1, a[r], 1 - a[r]) * np.where(E[:, r] == 1, e, 1 - e)
+        C[:, r] = A[:, r] * E[:, r]
+        prev = np.where(C[:, r] == 1, r, prev)
+    return C, p
+
+
+def pattern_probs(C, p):
+    """P(C = pattern) for every pattern, indexed by the bit mask sum_r c_r 2^r."""
+    L = C.shape[1]
+    code = (C.astype(np.int64) << np.arange(L)[None, :]).sum(axis=1)
+    return np.bincount(code, weights=p, minlength=1 << L)
+
+
+def conditional_from_patterns(pp, L):
+    """P(C_r = c_r | C_<r = c_<r) for every mask m (bit r = c_r) and rank r,
+    from the pattern probabilities pp: out[m, r]."""
+    masks = np.arange(1 << L)
+    out = np.zeros((1 << L, L))
+    for r in range(L):
+        upto = masks & ((1 << (r + 1)) - 1)
+        before = masks & ((1 << r) - 1)
+        p_upto = np.bincount(upto, weights=pp, minlength=1 << L)
+        p_before = np.bincount(before, weights=pp, minlength=1 << L)
+        out[:, r] = p_upto[upto] / p_before[before]
+    return out

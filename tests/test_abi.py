@@ -43,3 +43,23 @@ def test_memory_footprint_and_errors_without_device():
     st = L.mc_dataset_from_sessions(0, _lib.ptr(q, _lib.i32p), _lib.ptr(q, _lib.i32p), None, C.c_float(0.2),
                                     C.byref(ds))
     assert st == 1
+
+
+def test_session_arrays_are_shape_checked_before_native_calls():
+    """The Python mirror rejects session arrays whose shapes differ from the
+    element counts it passes to the C ABI (which trusts them)."""
+    import pytest
+    from massiveclicks_amd.model import SessionSet, _sessions_arrays
+    n = 4
+    ok = SessionSet(np.zeros(n, np.int32), np.zeros((n, 10), np.int32), np.zeros((n, 10), np.uint8),
+                    np.zeros((n, 10), np.int32), np.full(n, 10, np.uint8))
+    p, c, ln = _sessions_arrays(ok, "test")
+    assert p.shape == (n, 10) and c.shape == (n, 10) and ln.shape == (n,)
+    bad = [
+        SessionSet(ok.query, ok.docs, ok.clicks, np.zeros((n, 9), np.int32)),
+        SessionSet(ok.query, ok.docs, np.zeros((n - 1, 10), np.uint8), ok.pidx),
+        SessionSet(ok.query, ok.docs, ok.clicks, ok.pidx, np.full(n + 1, 10, np.uint8)),
+    ]
+    for b in bad:
+        with pytest.raises(ValueError):
+            _sessions_arrays(b, "test")

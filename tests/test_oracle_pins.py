@@ -19,9 +19,10 @@ import numpy as np
 import pytest
 
 import oracle
-from bruteforce import (atr_posterior_no_later_click, conditional_click_probs, factor_posteriors,
-                        factor_posteriors_fresh_tail, marginal_click_probs, pbm_posteriors,
-                        sat_posterior_last_click, ubm_marginal_click_probs)
+from bruteforce import (atr_posterior_no_later_click, conditional_click_probs, conditional_from_patterns,
+                        factor_posteriors, factor_posteriors_fresh_tail, independent_joint, marginal_click_probs,
+                        pattern_probs, pbm_posteriors, sat_posterior_last_click, ubm_joint,
+                        ubm_marginal_click_probs)
 
 PBM, CCM, DBN, UBM = 0, 1, 2, 3
 
@@ -154,6 +155,158 @@ def test_ubm_full_click_probs_match_enumeration():
     pc = [(lambda prev, r=r: float(a[r]) * float(e[ubm_expected_slot(r, prev)])) for r in range(10)]
     marg = ubm_marginal_click_probs(pc)
     np.testing.assert_allclose(1 - ev["full"][0], marg, rtol=1e-5, atol=1e-6)
+
+
+def all_masks_model(model, L, tables):
+    """An oracle model whose 2^L sessions show the same L pairs (indices
+    0..L-1) with every click pattern: session m clicks rank r iff bit r of m
+    is set.  Tables are set to the given values with denominator 1."""
+    n = 1 << L
+    clicks = np.zeros((n, 10), np.uint8)
+    clicks[:, :L] = (np.arange(n)[:, None] >> np.arange(L)[None, :]) & 1
+    pidx = np.zeros((n, 10), np.int32)
+    pidx[:, :L] = np.arange(L)
+    length = np.full(n, L, np.uint8)
+    om = oracle.OracleModel(model, np.zeros(n, np.int32), pidx, clicks, L, length=length)
+    for t, vals in tables.items():
+        vals = np.asarray(vals, np.float32)
+        om.set_table(t, vals, np.ones_like(vals))
+    return om, pidx, clicks, length
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 10])
+def test_pbm_conditional_and_full_click_probs_match_enumeration(L):
+    """PBM_Hst::get_log_conditional_click_probs and get_full_click_probs
+    (pbm.cu:376-439) over every click pattern, against enumeration of the
+    hidden (A_r, E_r): under the PBM the ranks are independent, so the
+    conditional and the full click probability of rank r are both
+    a_r e_r (click) or 1 - a_r e_r (no click), with e indexed by rank."""
+    rng = np.random.default_rng(40 + L)
+    a = rng.uniform(0.05, 0.95, L).astype(np.float32)
+    e = rng.uniform(0.05, 0.95, 10).astype(np.float32)
+    om, pidx, clicks, length = all_masks_model(PBM, L, {0: a, 1: e})
+    ev = om.evaluate(pidx, clicks, length)
+    C, p = independent_joint(a.astype(np.float64), e[:L].astype(np.float64))
+    cond = conditional_from_patterns(pattern_probs(C, p), L)
+    marg = (C * p[:, None]).sum(axis=0)
+    c = clicks[:, :L] == 1
+    np.testing.assert_allclose(np.exp(ev["logp"][:, :L].astype(np.float64)), cond, rtol=2e-6)
+    np.testing.assert_allclose(ev["full"][:, :L], np.where(c, marg[None, :], 1 - marg[None, :]), rtol=2e-6)
+    # A pair unseen in training (pidx -1) is valued PARAM_DEF_NUM/DENOM = 1/2.
+    p1 = pidx[:1].copy()
+    p1[0, L - 1] = -1
+    ev1 = om.evaluate(p1, clicks[:1], length[:1])
+    assert ev1["full"][0, L - 1] == pytest.approx(1 - 0.5 * float(e[L - 1]), rel=1e-6)
+
+
+def ubm_conditional_expected(a, e, slot_of, masks):
+    """a_r * e[slot_of(r, prev)] for the observed clicks (prev: last click
+    before r, -1 = none), as P(C_r = c_r | C_<r): [n_masks, 10]."""
+    out = np.zeros((len(masks), 10))
+    for i, m in enumerate(masks):
+        prev = -1
+        for r in range(10):
+            c = (m >> r) & 1
+            q = float(a[r]) * float(e[slot_of(r, prev, c)])
+            out[i, r] = q if c else 1 - q
+            if c:
+                prev = r
+    return out
+
+
+def test_ubm_conditional_click_probs_over_all_click_masks():
+    """UBM_Hst::get_log_conditional_click_probs (ubm.cu:373-401) on all 1024
+    click patterns of a ten-rank list, against enumeration of the hidden
+    (A_r, E_r) when E_r ~ Bern(e[slot]) with the slot of the explicit table
+    (row r(r+1)/2, column = previous click, 9 when there is none).  Two
+    mistaken slot rules fail this comparison, which shows that it tells
+    them apart: the previous click taken at or before rank r, and column 0
+    for "no earlier click"."""
+    rng = np.random.default_rng(11)
+    a = rng.uniform(0.05, 0.9, 10).astype(np.float32)
+    e = rng.uniform(0.05, 0.95, 55).astype(np.float32)
+    om, pidx, clicks, length = all_masks_model(UBM, 10, {0: a, 1: e})
+    ev = om.evaluate(pidx, clicks, length)
+    got = np.exp(ev["logp"].astype(np.float64))
+    ef = e.astype(np.float64)
+    C, p = ubm_joint(a.astype(np.float64), lambda r, prev: ef[ubm_expected_slot(r, prev)])
+    cond = conditional_from_patterns(pattern_probs(C, p), 10)
+    np.testing.assert_allclose(got, cond, rtol=2e-6)
+    masks = np.arange(1024)
+    right = ubm_conditional_expected(a, e, lambda r, prev, c: ubm_expected_slot(r, prev), masks)
+    np.testing.assert_allclose(right, cond, rtol=1e-12)
+    inclusive = ubm_conditional_expected(
+        a, e, lambda r, prev, c: r * (r + 1) // 2 + (r if c else (9 if prev == -1 else prev)), masks)
+    column0 = ubm_conditional_expected(
+        a, e, lambda r, prev, c: r * (r + 1) // 2 + (0 if prev == -1 else prev), masks)
+    for wrong in (inclusive, column0):
+        bad = np.abs(wrong - got) > 1e-3 * got
+        assert bad.any(axis=1).mean() > 0.5
+
+
+def test_ubm_full_click_probs_over_all_click_masks():
+    """UBM_Hst::get_full_click_probs (ubm.cu:412-468) on all 1024 click
+    patterns: P(C_r = 1) marginalised over every click history (from the
+    enumeration above), or one minus it for an unclicked rank."""
+    rng = np.random.default_rng(12)
+    a = rng.uniform(0.05, 0.9, 10).astype(np.float32)
+    e = rng.uniform(0.05, 0.95, 55).astype(np.float32)
+    om, pidx, clicks, length = all_masks_model(UBM, 10, {0: a, 1: e})
+    ev = om.evaluate(pidx, clicks, length)
+    ef = e.astype(np.float64)
+    C, p = ubm_joint(a.astype(np.float64), lambda r, prev: ef[ubm_expected_slot(r, prev)])
+    marg = (C * p[:, None]).sum(axis=0)
+    c = clicks == 1
+    np.testing.assert_allclose(ev["full"], np.where(c, marg[None, :], 1 - marg[None, :]), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_session_mean_and_totals_follow_evaluation(model):
+    """LogLikelihood::evaluate and Perplexity::evaluate
+    (evaluation.cpp:25-40, 49-66): a session's log-likelihood is the float
+    sum of its ranks' log conditional probabilities, from 0.f in rank order,
+    divided by its length (10 in the reference); the FP32 total adds the
+    sessions in order in float, the FP64 total in double; the perplexity
+    sums add log2 of the full click probability per rank.  Checked against
+    a recomputation from the per-rank outputs, for uneven lengths and
+    unseen pairs."""
+    rng = np.random.default_rng(50 + model)
+    n, n_qd = 400, 50
+    length = rng.integers(1, 11, n).astype(np.uint8)
+    pidx = rng.integers(0, n_qd, (n, 10)).astype(np.int32)
+    pidx[rng.random((n, 10)) < 0.1] = -1
+    clicks = (rng.random((n, 10)) < 0.3).astype(np.uint8)
+    for i in range(n):
+        clicks[i, length[i]:] = 0
+    tables = {0: rng.uniform(0.05, 0.9, n_qd)}
+    K = {PBM: 10, UBM: 55, DBN: 1, CCM: 3}[model]
+    if model == DBN:
+        tables[1] = rng.uniform(0.05, 0.9, n_qd)
+    tables[1 if model != DBN else 2] = rng.uniform(0.2, 0.95, K)
+    om = oracle.OracleModel(model, np.zeros(1, np.int32), np.zeros((1, 10), np.int32), np.zeros((1, 10), np.uint8),
+                            n_qd)
+    for t, v in tables.items():
+        v = np.asarray(v, np.float32)
+        om.set_table(t, v, np.ones_like(v))
+    ev = om.evaluate(pidx, clicks, length)
+    tot32 = np.float32(0)
+    tot64 = 0.0
+    pr64 = np.zeros(10)
+    for i in range(n):
+        L = int(length[i])
+        acc = np.float32(0)
+        for r in range(L):
+            acc = np.float32(acc + ev["logp"][i, r])
+        s = np.float32(acc / np.float32(L))
+        assert ev["llh"][i] == s, i
+        tot32 = np.float32(tot32 + s)
+        tot64 += float(s)
+        pr64[:L] += np.log2(ev["full"][i, :L].astype(np.float64))
+        assert np.all(ev["logp"][i, L:] == 0) and np.all(ev["full"][i, L:] == 0)
+    assert ev["llh_sum_f32"] == tot32
+    assert ev["llh_sum_f64"] == tot64
+    np.testing.assert_allclose(ev["ppl_log2_sum_f64"], pr64, rtol=1e-6)
+    np.testing.assert_array_equal(ev["rank_count"], [(length > r).sum() for r in range(10)])
 
 
 def _random_clicks_with_last(rng, L, lcr):
