@@ -8,12 +8,17 @@ HBM.  Prints one JSON line:
   roofline    the E/M item kernel (the dominant kernel): algorithmic bytes per
               launch / its mean duration (HIP events on the engine's stream)
               against the 8 TB/s HBM peak
-  cpu_baseline  the CPU oracle (reference-faithful threading) on a bounded
-              sample of the same workload, on this host's cores
+  heldout     held-out LL and perplexity from the resident test set, with
+              the evaluation kernel's duration and algorithmic bytes
+  cpu_baseline  the CPU oracle (the reference's threading) on the whole
+              training set, on 16 host threads (one GPU's share of the
+              host), without and with the reference's per-thread copy of
+              the training set
   time_to_target  DBN on 20M sessions: wall time of plain EM's 50
-              iterations, and of Adam on the log-likelihood gradient
-              (lr 0.05, minibatches of ~1M sessions) to reach EM's
-              50-iteration held-out log-likelihood within 1e-4 relative
+              iterations (and of its first iteration in the target band),
+              and of Adam on the log-likelihood gradient (lr 0.05,
+              minibatches of ~1M sessions) to reach EM's 50-iteration
+              held-out log-likelihood within 1e-4 relative
 
 Algorithmic bytes (per iteration): 42 B per training session (ten int32
 query-document indices + a 16-bit click mask) + 16 B per table entry and
@@ -39,46 +44,91 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ds, model, n_sample, iters, threads):
-    """Times the CPU oracle (reference-faithful threading: even E-step split,
-    query-aligned M-step ranges, mutex merge of the shared table) on a uniform
-    random sample of the training sessions, kept in query order."""
-    import numpy as np
+def cpu_model_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# Published CPU comparator of the reference (DBN, 10M sessions, 48 threads,
+# EPYC 7402P): experiments/data/model_2/EPYC 7402P/nodes_1/
+# sessions_10000000/partitioning_2/iteration_50.txt:105.
+PUBLISHED_CPU_S_PER_ITER = 17.52
+PUBLISHED_CPU_THREADS = 48
+SERP_HST_BYTES = 124     # one SERP_Hst: int query + 10 x (click, doc, param index)
+
+
+def cpu_baseline(ds, model, iters, copy_iters, threads):
+    """Times the CPU oracle in the reference's threading (even E-step split,
+    query-aligned M-step ranges, mutex merge of the shared tables;
+    parallel_em.cu:27-197) on the whole training set: `iters` iterations as
+    is, then `copy_iters` iterations that also replay the DBN/CCM E-step's
+    copy of the training set into every thread (std::thread takes the train
+    set by value, dbn.cu:244, ccm.cu:223)."""
     import oracle
     tr = ds.train
-    n = min(n_sample, len(tr))
-    idx = np.sort(np.random.default_rng(0).choice(len(tr), n, replace=False))
-    q = tr.query[idx]
-    p = tr.pidx[idx]
-    n_qd = int(p.max()) + 1
-    ln = None if tr.length is None else tr.length[idx]
-    om = oracle.OracleModel(model, q, p, tr.clicks[idx], n_qd, length=ln, n_threads=threads)
+    n = len(tr)
+    om = oracle.OracleModel(model, tr.query, tr.pidx, tr.clicks, ds.n_qd, length=tr.length, n_threads=threads)
+
+    def timed(k):
+        te = tm = 0.0
+        for _ in range(k):
+            t0 = time.perf_counter()
+            om.estep()
+            t1 = time.perf_counter()
+            om.mstep()
+            t2 = time.perf_counter()
+            te += t1 - t0
+            tm += t2 - t1
+        return te / k, tm / k
+
     om.iterate(1)   # warm
-    te = tm = 0.0
-    for _ in range(iters):
-        t0 = time.perf_counter()
-        om.estep()
-        t1 = time.perf_counter()
-        om.mstep()
-        t2 = time.perf_counter()
-        te += t1 - t0
-        tm += t2 - t1
-    dt = (te + tm) / iters
+    e, m = timed(iters)
+    dt = e + m
     om.close()
-    return {"value": n / dt, "unit": "sessions/s/iter", "cores": threads, "kind": "port",
-            "sample": f"{n} of the training sessions (uniform sample, query order kept) x {iters} iterations, "
-                      f"{threads} threads: {dt:.3f} s/iter (E {te / iters:.3f} s, M {tm / iters:.3f} s)"}
+    out = {"value": n / dt, "unit": "sessions/s/iter", "cores": threads, "kind": "port",
+           "cpu": cpu_model_name(), "host_cpus": os.cpu_count(),
+           "sample": f"all {n} training sessions x {iters} iterations, {threads} threads: {dt:.3f} s/iter "
+                     f"(E {e:.3f} s, M {m:.3f} s)"}
+    if copy_iters and model in (1, 2):
+        om = oracle.OracleModel(model, tr.query, tr.pidx, tr.clicks, ds.n_qd, length=tr.length, n_threads=threads,
+                                copy_dataset=True)
+        ec, mc = timed(copy_iters)
+        om.close()
+        dc = ec + mc
+        per_copy = max(0.0, dc - dt) / threads
+        # The same run on the published comparator's 48 threads: the
+        # E-step's compute spread over 48 threads, 48 serial copies.
+        est48 = e * threads / PUBLISHED_CPU_THREADS + m + PUBLISHED_CPU_THREADS * per_copy
+        out.update({
+            "value_with_copy": n / dc,
+            "with_copy": f"{copy_iters} iterations with the per-thread train-set copy "
+                         f"({threads} x {n * SERP_HST_BYTES / 1e9:.2f} GB per iteration): {dc:.3f} s/iter "
+                         f"(E {ec:.3f} s, M {mc:.3f} s), {per_copy:.3f} s per copy",
+            "published_reference": f"{PUBLISHED_CPU_S_PER_ITER} s/iter on {PUBLISHED_CPU_THREADS} threads "
+                                   f"(EPYC 7402P, Yandex 10M); this host's figures with {PUBLISHED_CPU_THREADS} "
+                                   f"copies per iteration: about {est48:.2f} s/iter",
+        })
+    return out
 
 
 ADAM_ITEM_SESSIONS = 4096
 
 
 def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
-    """Wall time for plain EM (50 iterations) and for Adam on the
-    log-likelihood gradient to reach the held-out log-likelihood of EM's
-    50th iteration (within 1e-4 relative), DBN.  EM iterations are replayed
-    from one graph and timed with HIP events; Adam epochs (every minibatch
-    once) likewise; held-out evaluations between epochs are not timed."""
+    """Wall time for plain EM and for Adam on the log-likelihood gradient to
+    reach the held-out log-likelihood of EM's 50th iteration (within 1e-4
+    relative), DBN.  EM: 50 iterations replayed from one graph and timed
+    with HIP events; its held-out LL is also evaluated after every single
+    iteration (untimed), which gives the first iteration that reaches the
+    target band and its share of the 50-iteration time.  Adam: epochs
+    (every minibatch once) replayed from one graph; held-out evaluations
+    between epochs are not timed."""
     from massiveclicks_amd import Dataset, ClickModel, synth
     from massiveclicks_amd.model import DBN
     t0 = time.perf_counter()
@@ -90,11 +140,17 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
     test = ds.test
     with ClickModel(DBN) as cm:
         cm.init_from_dataset(ds)
+        cm.set_test_set(test)
         em_ms = cm.em_graph_timed(50)      # one warm replay of 50 iterations from the start, then 50 timed
         cm.reset_parameters()
-        cm.em(50)
-        target = cm.evaluate(test)["log_likelihood"]
+        em_curve = []
+        for it in range(1, 51):
+            cm.em(1)
+            em_curve.append(cm.evaluate_test_set()["log_likelihood"])
+        target = em_curve[-1]
     thr = target - 1e-4 * abs(target)
+    em_first = next(i + 1 for i, ll in enumerate(em_curve) if ll >= thr)
+    em_first_ms = em_ms * em_first / 50
     batches = max(1, round(ds.n_train / batch_sessions))
     curve = []
     reached = None
@@ -103,29 +159,35 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
     # (8192, the default, for EM).
     with ClickModel(DBN, batches=batches, item_sessions=ADAM_ITEM_SESSIONS) as cm:
         cm.init_from_dataset(ds)
+        cm.set_test_set(test)
         cm.grad_init()
         cm.adam_configure(lr=lr)
         total = 0.0
         for ep in range(1, max_epochs + 1):
             total += cm.adam_steps_timed(batches)
-            ll = cm.evaluate(test)["log_likelihood"]
+            ll = cm.evaluate_test_set()["log_likelihood"]
             curve.append([ep, round(total, 3), round(ll, 6)])
             if ll >= thr:
                 reached = (ep, total, ll)
                 break
-    log(f"time-to-target: EM 50 it {em_ms:.2f} ms -> LL {target:.6f}; Adam {curve[-1]}")
+    log(f"time-to-target: EM 50 it {em_ms:.2f} ms -> LL {target:.6f}, first in band at iteration {em_first} "
+        f"({em_first_ms:.3f} ms); Adam {curve[-1]}")
     return {
         "workload": f"DBN, {n_sessions} synthetic sessions ({ds.n_train} train), 1M Zipf(1.1) queries; Adam lr {lr}, "
                     f"{batches} minibatches of whole query groups (~{batch_sessions} sessions), log-prior on, "
                     f"{ADAM_ITEM_SESSIONS}-session work items (EM: 8192)",
         "target_heldout_ll": target,
-        "threshold": "held-out LL within 1e-4 relative of EM's 50-iteration LL",
+        "threshold": "held-out LL at least EM's 50-iteration LL minus 1e-4 relative",
         "em_ms": em_ms,
+        "em_first_iteration_in_band": em_first,
+        "em_first_ms": em_first_ms,
+        "em_curve": [round(x, 6) for x in em_curve],
         "adam_ms": reached[1] if reached else None,
         "adam_epochs": reached[0] if reached else None,
         "adam_steps": reached[0] * batches if reached else None,
         "adam_ll": reached[2] if reached else curve[-1][2],
         "speedup_vs_em": (em_ms / reached[1]) if reached else None,
+        "speedup_vs_em_first": (em_first_ms / reached[1]) if reached else None,
         "adam_curve": curve,
     }
 
@@ -179,9 +241,9 @@ def main():
     ap.add_argument("--item-window", type=int, default=None)
     ap.add_argument("--model", default="dbn", choices=["pbm", "ubm", "dbn", "ccm"])
     ap.add_argument("--mode", default="segmented", choices=["segmented", "atomic"])
-    # About 10 s of CPU work on 16 threads.
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
-    ap.add_argument("--cpu-iters", type=int, default=5)
+    # About 5 s of CPU work on 16 threads without the copy, 10 s with it.
+    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--cpu-copy-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--item-sessions", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
@@ -275,19 +337,35 @@ def main():
     layout_sess = n_w * (2 * mean_len + extra) + (n_train - n_w) * (4 * mean_len + extra)
     layout_bytes = layout_sess + (8 * n_tab + 2) * (ds.n_qd - n_shared) if segmented else layout_sess
 
-    ev = cm.evaluate(ds.test)
-    log(f"held-out LL {ev['log_likelihood']:.6f}, perplexity {ev['perplexity']:.6f}")
+    # Held-out evaluation on the resident test set: the evaluation kernel's
+    # duration (HIP events) against its algorithmic bytes: per session the
+    # 32-bit index of every shown rank, the 2-byte click mask (+1 B length)
+    # and one 4-byte value gather per shown rank and per-qd table.
+    cm.set_test_set(ds.test)
+    cm.evaluate_test_set()
+    ev = min((cm.evaluate_test_set(timed=True) for _ in range(5)), key=lambda x: x["kernel_ms"])
+    te_len = float(ds.test.length.mean()) if ds.test.length is not None else 10.0
+    ev_bytes = ds.n_test * (4 * te_len + 2 + (1 if batched else 0) + 4 * n_tab * te_len)
+    log(f"held-out LL {ev['log_likelihood']:.6f}, perplexity {ev['perplexity']:.6f}, "
+        f"eval kernel {ev['kernel_ms']:.4f} ms")
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"{args.model}_{args.sessions // 1_000_000}m_traffic.json")
+    # HBM traffic per launch from the calibrated FETCH_SIZE / WRITE_SIZE
+    # counter runs of this command (rocprofv3 cannot run inside it); the
+    # profile's name travels with the figure.
+    traffic, traffic_src = None, None
+    tname = f"{args.model}_{args.sessions // 1_000_000}m_traffic.json"
+    tfile = os.path.join(ROOT, "profiles", tname)
     if os.path.exists(tfile) and args.mode == "segmented" and not batched and not args.autotune:
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
+        traffic_src = f"profiles/{tname} ({tj.get('kernel')}, measured {tj.get('measured', 'see file')})"
 
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(ds, model, args.cpu_sample, args.cpu_iters, threads)
+        # The GPU box's CPU share for one GPU is 16 threads.
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        cpu = cpu_baseline(ds, model, args.cpu_iters, args.cpu_copy_iters, threads)
         log(f"cpu baseline: {cpu}")
 
     out = {
@@ -315,7 +393,7 @@ def main():
                    "iterations": args.steps, "options": {k: v for k, v in opts.items()},
                    "work_items": lay["n_items"], "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": ("em_window_persistent_kernel" if segmented and opts.get("persistent", True)
                                 and opts.get("block", 256) == 256 else "em_item_kernel"),
                      "bytes_per_launch": item_bytes,
@@ -326,7 +404,11 @@ def main():
                      "iteration_bytes": iter_bytes,
                      "iteration_frac": iter_bytes / (ms_iter * 1e-3) / HBM_PEAK},
         "cpu_baseline": cpu,
-        "heldout": {"log_likelihood": ev["log_likelihood"], "perplexity": ev["perplexity"]},
+        "heldout": {"log_likelihood": ev["log_likelihood"], "perplexity": ev["perplexity"],
+                    "test_sessions": ds.n_test,
+                    "eval_kernel": "eval_kernel (resident test set)", "eval_kernel_ms": ev["kernel_ms"],
+                    "eval_bytes": ev_bytes,
+                    "eval_frac": ev_bytes / (ev["kernel_ms"] * 1e-3) / HBM_PEAK},
     }
     cm.close()
     del ds

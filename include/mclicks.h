@@ -186,6 +186,32 @@ mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t
                       float* logp, float* full, float* llh, double* llh_sum, double* ppl_log2_sum,
                       int64_t* rank_count);
 
+/* get_serp_probability (base.cuh:50; pbm.cu:351-365, ubm.cu:345-362,
+ * dbn.cu:572-600, ccm.cu:566-593) for n sessions: prob[n*10] receives, per
+ * rank, a*e for the PBM and UBM (the click probability given the earlier
+ * clicks, whatever this rank's click) and, for the DBN and CCM, the
+ * probability of this rank's observed outcome given the earlier clicks
+ * (the value whose log mc_evaluate returns in logp).  Arguments as in
+ * mc_evaluate; ranks past len[i] receive 0. */
+mc_status mc_serp_probability(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                              float* prob);
+
+/* The held-out set of the evaluation in parallel_em.cu:694-724 (the test
+ * set LogLikelihood::evaluate and Perplexity::evaluate are given), uploaded
+ * once and kept on the device: arguments as in mc_evaluate.  The sessions
+ * are stored in the order of their smallest pair index, which puts the
+ * sessions of one query next to each other, with the permutation back to
+ * the caller's order.  A later call replaces the set; mc_init_parameters
+ * drops it. */
+mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len);
+
+/* mc_evaluate on the resident held-out set at the current parameters.
+ * Per-session outputs (nullable) are in the order the set was given.
+ * kernel_ms (nullable) receives the evaluation kernel's duration (HIP events
+ * on the model's stream). */
+mc_status mc_evaluate_test_set(mc_model* m, float* logp, float* full, float* llh, double* llh_sum,
+                               double* ppl_log2_sum, int64_t* rank_count, double* kernel_ms);
+
 /* get_memory_usage (base.cuh:31): device bytes held by the model. */
 mc_status mc_get_memory_usage(const mc_model* m, size_t* bytes);
 

@@ -206,6 +206,29 @@ public:
     // LogLikelihood + Perplexity (evaluation.cpp:25-66), totals in FP64.
     Evaluation evaluate(const Sessions& s) { return run_eval(s, nullptr, nullptr, nullptr); }
 
+    // get_serp_probability (base.cuh:50) for a whole set: [n * 10].
+    void get_serp_probability(const Sessions& s, std::vector<float>& out) {
+        out.resize((size_t)s.size() * MC_MAX_SERP);
+        if (!s.size()) return;
+        check(mc_serp_probability(m_.get(), s.size(), s.pidx.data(), s.clicks.data(),
+                                  s.length.empty() ? nullptr : s.length.data(), out.data()));
+    }
+
+    // The held-out set kept on the device (mc_set_test_set), then evaluated
+    // at the current parameters as often as needed (mc_evaluate_test_set).
+    void set_test_set(const Sessions& s) {
+        check(mc_set_test_set(m_.get(), s.size(), s.pidx.data(), s.clicks.data(),
+                              s.length.empty() ? nullptr : s.length.data()));
+        n_test_ = s.size();
+    }
+    Evaluation evaluate_test_set() {
+        Evaluation e;
+        e.n_sessions = n_test_;
+        check(mc_evaluate_test_set(m_.get(), nullptr, nullptr, nullptr, &e.llh_sum, e.ppl_log2_sum.data(),
+                                   e.rank_count.data(), nullptr));
+        return e;
+    }
+
     size_t get_memory_usage() const {
         size_t b = 0;
         check(mc_get_memory_usage(m_.get(), &b));
@@ -289,6 +312,7 @@ private:
         return e;
     }
     std::unique_ptr<mc_model, Deleter> m_;
+    int64_t n_test_ = 0;
 };
 
 }  // namespace mclicks

@@ -18,7 +18,7 @@ CLI = os.path.join(LIBDIR, "mclicks")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mc_engine.hip", "mc_grad.hip", "mc_dataset.cpp", "mc_io.cpp", "mc_tune.cpp"]
+SOURCES = ["mc_engine.hip", "mc_eval.hip", "mc_grad.hip", "mc_dataset.cpp", "mc_io.cpp", "mc_tune.cpp"]
 
 # -ffp-contract=off keeps the E-step arithmetic in the reference's operation
 # order; packed FP32 instructions are disabled on the device.
@@ -44,10 +44,18 @@ def _run(cmd, what, verbose):
 
 
 def build(verbose: bool = False) -> str:
-    """Builds lib/libmclicks.so and the command-line driver lib/mclicks."""
-    os.makedirs(LIBDIR, exist_ok=True)
+    """Builds lib/libmclicks.so and the command-line driver lib/mclicks.
+    Every source compiles to its own object, in parallel, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    jobs = max(1, min(len(SOURCES), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(lambda so: _run([HIPCC, *FLAGS, "-c", "-o", so[1], os.path.join(CSRC, so[0])], so[0], verbose),
+                      zip(SOURCES, objs)))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *FLAGS, "-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES], "libmclicks.so", verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp] + objs, "libmclicks.so", verbose)
     os.replace(tmp, LIB)
     tmp = CLI + ".tmp"
     _run([HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "mc_cli.cpp"),

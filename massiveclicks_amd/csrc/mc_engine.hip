@@ -1,6 +1,7 @@
 // mc_engine.hip -- kernels and C ABI of the MI355X EM engine.
 //
-// Per EM iteration two kernels run on the model's stream:
+// Per EM iteration two kernels run on the model's stream (the held-out
+// evaluation is in mc_eval.hip, the gradient path in mc_grad.hip):
 //   em_item_kernel   E-step of one work item per workgroup, per-item
 //                    reduction of the expected counts, in-place M-step of the
 //                    entries the item owns (parallel_em/kernel.cu:42-89 and
@@ -194,7 +195,12 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     constexpr bool win = PATH == kPathLds || fwin;
     unsigned long long* lacc = lds;
     float* facc = (float*)lds;
-    float* lval = fwin ? facc + T * W : (float*)(lds + T * W);
+    // Fixed-point windows take max_width slots per table (copies of the
+    // entries of narrow windows), then the staged values.
+    const int Wmax = A.max_width;
+    const AccWindow<T> aw{lacc, Wmax, PATH == kPathLds ? acc_copies_log2(W, Wmax) : 0};
+    const int WS = W << aw.rsh;
+    float* lval = fwin ? facc + T * W : (float*)(lds + T * Wmax);
 
     // First session's loads go out before the window is staged.
     const Stream st = make_stream(A);
@@ -208,10 +214,13 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 #pragma unroll
             for (int t = 0; t < T; t++) {
                 if (fwin) facc[i * T + t] = 0.f;
-                else lacc[i * T + t] = 0ull;
                 lval[i * T + t] = A.tab[t].val_in[it.base + i];
             }
         }
+        if (!fwin)
+            for (int i = tid; i < WS; i += nthr)
+#pragma unroll
+                for (int t = 0; t < T; t++) lacc[aw.slot(t, i)] = 0ull;
     }
     __syncthreads();
 
@@ -224,7 +233,7 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
         EmitGlobalFloat em{A.tab[0].gaccf, A.tab[1].gaccf};
         run_item<MODEL, VARLEN, false>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLds) {
-        EmitLds<T> em{lacc};
+        EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1)};
         const SrcLds<T> lsrc{lval};
         run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLdsFloat) {
@@ -281,7 +290,7 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
                 const int2 oc = interior ? unpack_counts(pk[u], A.counts, p) : make_int2(0, 0);
 #pragma unroll
                 for (int t = 0; t < T; t++) {
-                    const unsigned long long v = lacc[i * T + t];
+                    const unsigned long long v = aw.sum(t, i);
                     if (interior) {
                         const float2 bd = base_den<MODEL>(t, oc);
                         const float num = fix_to_float(bd.x, v);
@@ -362,16 +371,19 @@ em_window_persistent_kernel(KArgs A, int n_win) {
 
     while (cur < n_win) {
         const int W = it.width;
+        const AccWindow<T> aw{lacc, Wmax, acc_copies_log2(W, Wmax)};
+        const int WS = W << aw.rsh;
         // Stage the window, take the next item.
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int i = u * NT + tid;
             if (i < W) {
 #pragma unroll
-                for (int t = 0; t < T; t++) {
-                    lacc[i * T + t] = 0ull;
-                    lval[i * T + t] = wv[u][t];
-                }
+                for (int t = 0; t < T; t++) lval[i * T + t] = wv[u][t];
+            }
+            if (i < WS) {
+#pragma unroll
+                for (int t = 0; t < T; t++) lacc[aw.slot(t, i)] = 0ull;
             }
         }
         __syncthreads();
@@ -395,7 +407,7 @@ em_window_persistent_kernel(KArgs A, int n_win) {
             cnt[u] = (i < W) ? A.counts16[it.base + i] : 0u;
         }
         {
-            EmitLds<T> em{lacc};
+            EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1)};
             const SrcLds<T> lsrc{lval};
             run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
         }
@@ -419,7 +431,7 @@ em_window_persistent_kernel(KArgs A, int n_win) {
             const int2 oc = interior ? unpack_counts(cnt[u], A.counts, p) : make_int2(0, 0);
 #pragma unroll
             for (int t = 0; t < T; t++) {
-                const unsigned long long v = lacc[i * T + t];
+                const unsigned long long v = aw.sum(t, i);
                 if (interior) {
                     const float2 bd = base_den<MODEL>(t, oc);
                     const float num = fix_to_float(bd.x, v);
@@ -502,212 +514,6 @@ __global__ void __launch_bounds__(256) em_merge_kernel(KArgs A) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Evaluation kernels (pbm.cu:376-439, ubm.cu:373-468, dbn.cu:611-683,
-// ccm.cu:604-671; evaluation.cpp:25-66).  pidx -1 = pair unseen in training,
-// valued PARAM_DEF_NUM / PARAM_DEF_DENOM.
-// ---------------------------------------------------------------------------
-struct EvalArgs {
-    const int32_t* pidx;     // [n][10]
-    const uint8_t* clicks;   // [n][10]
-    const uint8_t* len;      // [n] or null
-    int64_t n;
-    const float* val0;       // attractiveness values
-    const float* val1;       // satisfaction values (DBN)
-    const float* sh;         // shared values
-    float* logp;             // [n][10] or null
-    float* full;             // [n][10] or null
-    float* llh;              // [n] or null
-    double* partial;         // [n_blocks][21]: llh, ppl[10], count[10]
-};
-
-__device__ __forceinline__ float pv_or_default(const float* v, int p) {
-    return p != -1 ? v[p] : 0.5f;   // PARAM_DEF_NUM / PARAM_DEF_DENOM
-}
-
-// One test session.  All of its loads are issued before any arithmetic: the
-// ten indices as five 8-byte loads (rows are 40 B, so 8-byte aligned), the
-// click bytes as five 2-byte loads, then the value gathers of every shown
-// rank; the chains then run from registers, unrolled over the ten ranks with
-// r < L guards.  The arithmetic is the reference's, operation for operation.
-template <int MODEL>
-__device__ void eval_session(const EvalArgs& E, int64_t i, int L, float (&lp)[kL], float (&fp)[kL]) {
-    int p[kL];
-    const int2* p2 = reinterpret_cast<const int2*>(E.pidx + i * kL);
-#pragma unroll
-    for (int u = 0; u < kL / 2; u++) {
-        const int2 v = p2[u];
-        p[2 * u] = v.x;
-        p[2 * u + 1] = v.y;
-    }
-    bool c[kL];
-    const uint16_t* c2 = reinterpret_cast<const uint16_t*>(E.clicks + i * kL);
-#pragma unroll
-    for (int u = 0; u < kL / 2; u++) {
-        const uint16_t v = c2[u];
-        c[2 * u] = (v & 0xffu) == 1;
-        c[2 * u + 1] = (v >> 8) == 1;
-    }
-    float a[kL], sv[kL];
-#pragma unroll
-    for (int r = 0; r < kL; r++) {
-        a[r] = r < L ? pv_or_default(E.val0, p[r]) : 0.5f;
-        sv[r] = (MODEL == kDBN && r < L) ? pv_or_default(E.val1, p[r]) : 0.5f;
-    }
-    if (MODEL == kPBM) {
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                float ae = a[r] * E.sh[r];
-                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
-                fp[r] = c[r] ? ae : 1 - ae;
-            }
-        }
-    } else if (MODEL == kUBM) {
-        int prev[kL];
-        int last = kL - 1;
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            prev[r] = last;
-            if (r < L && c[r]) last = r;
-        }
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                float ae = a[r] * E.sh[r * (r + 1) / 2 + prev[r]];
-                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
-            }
-        }
-        // P(C_r = 1) summed over the rank of the previous click (-1: none).
-        float tf[kL];
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                float cp = 0.f;
-#pragma unroll
-                for (int rp = -1; rp < r; rp++) {
-                    float nb = 1.f;
-                    const int crp = rp == -1 ? 9 : rp;
-#pragma unroll
-                    for (int rb = rp + 1; rb < r; rb++) nb *= 1 - a[rb] * E.sh[rb * (rb + 1) / 2 + crp];
-                    float temp = nb * (a[r] * E.sh[r * (r + 1) / 2 + crp]);
-                    cp += rp >= 0 ? tf[rp] * temp : temp;
-                }
-                tf[r] = cp;
-                fp[r] = c[r] ? cp : 1 - cp;
-            } else {
-                tf[r] = 0.f;
-            }
-        }
-    } else if (MODEL == kDBN) {
-        const float g = E.sh[0];
-        float ex = 1.f;
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                float cp;
-                if (c[r]) { cp = a[r] * ex; ex = g * (1 - sv[r]); }
-                else { cp = 1 - a[r] * ex; ex *= g * (1 - a[r]) / cp; }
-                lp[r] = logf(cp);
-            }
-        }
-        ex = 1.f;
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                float ae = a[r] * ex;
-                fp[r] = c[r] ? ae : 1 - ae;
-                ex *= g * (1 - a[r]) + g * a[r] * (1 - sv[r]);
-            }
-        }
-    } else {
-        const float t1 = E.sh[0], t2 = E.sh[1], t3 = E.sh[2];
-        float ex = 1.f;
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                float cp;
-                if (c[r]) { cp = a[r] * ex; ex = t2 * (1 - a[r]) + t3 * a[r]; }
-                else { cp = 1 - a[r] * ex; ex *= t1 * (1 - a[r]) / cp; }
-                lp[r] = logf(cp);
-            }
-        }
-        ex = 1.f;
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                float ae = a[r] * ex;
-                fp[r] = c[r] ? ae : 1 - ae;
-                ex *= (1 - a[r]) * t1 + a[r] * ((1 - a[r]) * t2 + a[r] * t3);
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-template <int MODEL>
-__global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
-    __shared__ double red[21][4];
-    const int tid = threadIdx.x;
-    double v[21];
-#pragma unroll
-    for (int k = 0; k < 21; k++) v[k] = 0.0;
-    // Grid-stride: each lane sums several sessions before the block reduction.
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < E.n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int L = E.len ? (int)E.len[i] : kL;
-        float lp[kL], fp[kL];
-        eval_session<MODEL>(E, i, L, lp, fp);
-        float acc = 0.f;
-#pragma unroll
-        for (int r = 0; r < kL; r++)
-            if (r < L) acc = acc + lp[r];
-        const float s = acc / L;                 // evaluation.cpp:36
-        v[0] += (double)s;
-        if (E.llh) E.llh[i] = s;
-#pragma unroll
-        for (int r = 0; r < kL; r++) {
-            if (r < L) {
-                v[1 + r] += (double)log2f(fp[r]);   // evaluation.cpp:62
-                v[11 + r] += 1.0;
-            }
-            if (E.logp) E.logp[i * kL + r] = r < L ? lp[r] : 0.f;
-            if (E.full) E.full[i * kL + r] = r < L ? fp[r] : 0.f;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 21; k++) {
-        double w = wave_sum_d(v[k]);
-        if ((tid & 63) == 0) red[k][tid >> 6] = w;
-    }
-    __syncthreads();
-    if (tid < 21) {
-        double w = 0.0;
-        for (int j = 0; j < (int)(blockDim.x >> 6); j++) w += red[tid][j];
-        E.partial[(int64_t)blockIdx.x * 21 + tid] = w;
-    }
-}
-
-// Sums block partials: one block per output, each thread a fixed strided
-// subset, then a fixed tree; the result does not depend on scheduling.
-__global__ void __launch_bounds__(256) eval_reduce_kernel(const double* partial, int n_blocks, double* out) {
-    __shared__ double red[256];
-    const int k = blockIdx.x, tid = threadIdx.x;
-    double w = 0.0;
-    for (int b = tid; b < n_blocks; b += 256) w += partial[(int64_t)b * 21 + k];
-    red[tid] = w;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (tid < off) red[tid] += red[tid + off];
-        __syncthreads();
-    }
-    if (tid == 0) out[k] = red[0];
-}
-
 // Fills a float array with a constant (used instead of memsets so that the
 // same code is safe inside captured graphs).
 __global__ void fill_kernel(float* p, float v, int64_t n) {
@@ -742,6 +548,13 @@ void release(mc_model* m) {
     for (void* p : m->allocations) (void)hipFree(p);
     m->allocations.clear();
     m->mem_bytes = 0;
+    m->d_test_pidx = nullptr;
+    m->d_test_mask = nullptr;
+    m->d_test_len = nullptr;
+    m->d_test_perm = nullptr;
+    m->d_test_part = nullptr;
+    m->n_test = 0;
+    m->test_bytes = 0;
     m->initialized = false;
 }
 
@@ -1743,89 +1556,6 @@ mc_status mc_reset_parameters(mc_model* m) {
     MC_TRY(fill(m, m->d_sh_den[m->cur], 2.f, m->K));
     MC_HIP(hipStreamSynchronize(m->stream));
     m->estep_pending = false;
-    return MC_OK;
-}
-
-mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
-                      float* logp, float* full, float* llh, double* llh_sum, double* ppl_log2_sum,
-                      int64_t* rank_count) {
-    MC_REQUIRE(m && m->initialized, "model not initialised");
-    MC_REQUIRE(n >= 1 && pidx && clicks, "invalid argument");
-    MC_HIP(hipSetDevice(m->device));
-    for (int64_t i = 0; i < n; i++) {
-        const int l = len ? len[i] : kMaxSerp;
-        MC_REQUIRE(l >= 1 && l <= kMaxSerp, "len must be in [1, 10]");
-        for (int r = 0; r < l; r++) {
-            const int32_t p = pidx[i * kMaxSerp + r];
-            MC_REQUIRE(p >= -1 && p < m->n_qd, "test pidx out of [-1, n_qd)");
-            MC_REQUIRE(clicks[i * kMaxSerp + r] <= 1, "clicks must be 0 or 1");
-        }
-    }
-    // Temporary device buffers for this call.
-    const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
-    std::vector<void*> tmp;
-    auto talloc = [&](size_t bytes) -> void* {
-        void* p = nullptr;
-        if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
-        tmp.push_back(p);
-        return p;
-    };
-    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
-    int32_t* d_p = (int32_t*)talloc((size_t)n * kMaxSerp * 4);
-    uint8_t* d_c = (uint8_t*)talloc((size_t)n * kMaxSerp);
-    uint8_t* d_l = len ? (uint8_t*)talloc((size_t)n) : nullptr;
-    float* d_lp = logp ? (float*)talloc((size_t)n * kMaxSerp * 4) : nullptr;
-    float* d_fp = full ? (float*)talloc((size_t)n * kMaxSerp * 4) : nullptr;
-    float* d_llh = llh ? (float*)talloc((size_t)n * 4) : nullptr;
-    double* d_part = (double*)talloc((size_t)nb * 21 * 8);
-    double* d_out = (double*)talloc(21 * 8);
-    if (!d_p || !d_c || (len && !d_l) || (logp && !d_lp) || (full && !d_fp) || (llh && !d_llh) || !d_part || !d_out) {
-        cleanup();
-        set_error("device allocation failed in mc_evaluate");
-        return MC_ERR_OOM;
-    }
-    hipError_t e = hipSuccess;
-    auto chk = [&](hipError_t x) { if (e == hipSuccess && x != hipSuccess) e = x; };
-    chk(hipMemcpyAsync(d_p, pidx, (size_t)n * kMaxSerp * 4, hipMemcpyHostToDevice, m->stream));
-    chk(hipMemcpyAsync(d_c, clicks, (size_t)n * kMaxSerp, hipMemcpyHostToDevice, m->stream));
-    if (len) chk(hipMemcpyAsync(d_l, len, (size_t)n, hipMemcpyHostToDevice, m->stream));
-    EvalArgs E;
-    E.pidx = d_p;
-    E.clicks = d_c;
-    E.len = d_l;
-    E.n = n;
-    E.val0 = m->d_val[m->cur][0];
-    E.val1 = m->T > 1 ? m->d_val[m->cur][1] : m->d_val[m->cur][0];
-    E.sh = m->d_sh_val[m->cur];
-    E.logp = d_lp;
-    E.full = d_fp;
-    E.llh = d_llh;
-    E.partial = d_part;
-    switch (m->type) {
-        case MC_PBM: hipLaunchKernelGGL(eval_kernel<kPBM>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
-        case MC_UBM: hipLaunchKernelGGL(eval_kernel<kUBM>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
-        case MC_DBN: hipLaunchKernelGGL(eval_kernel<kDBN>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
-        default: hipLaunchKernelGGL(eval_kernel<kCCM>, dim3((unsigned)nb), dim3(256), 0, m->stream, E); break;
-    }
-    chk(hipGetLastError());
-    hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, d_part, (int)nb, d_out);
-    chk(hipGetLastError());
-    double out[21];
-    chk(hipMemcpyAsync(out, d_out, 21 * 8, hipMemcpyDeviceToHost, m->stream));
-    if (logp) chk(hipMemcpyAsync(logp, d_lp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
-    if (full) chk(hipMemcpyAsync(full, d_fp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
-    if (llh) chk(hipMemcpyAsync(llh, d_llh, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream));
-    chk(hipStreamSynchronize(m->stream));
-    cleanup();
-    if (e != hipSuccess) {
-        set_error(std::string("mc_evaluate: ") + hipGetErrorString(e));
-        return MC_ERR_HIP;
-    }
-    if (llh_sum) *llh_sum = out[0];
-    for (int r = 0; r < kMaxSerp; r++) {
-        if (ppl_log2_sum) ppl_log2_sum[r] = out[1 + r];
-        if (rank_count) rank_count[r] = (int64_t)out[11 + r];
-    }
     return MC_OK;
 }
 

@@ -1,0 +1,549 @@
+// mc_eval.hip -- held-out evaluation on the device: per-session conditional
+// and full click probabilities (pbm.cu:376-439, ubm.cu:373-468,
+// dbn.cu:611-683, ccm.cu:604-671), the SERP click probabilities
+// (get_serp_probability, pbm.cu:351, ubm.cu:345, dbn.cu:572, ccm.cu:566),
+// and the log-likelihood and perplexity totals (evaluation.cpp:25-66) in
+// FP64.
+//
+// Two session layouts feed the same per-session arithmetic:
+//   * the caller's arrays, session-major ([n][10] int32 pair indices and
+//     [n][10] click bytes), uploaded per call (mc_evaluate,
+//     mc_serp_probability);
+//   * a held-out set kept resident on the device (mc_set_test_set): sessions
+//     reordered so that the sessions of one query are adjacent (their value
+//     gathers then share cache lines), stored rank-major ([10][n] int32 pair
+//     indices, 16-bit click masks), with the permutation back to the
+//     caller's order for the per-session outputs.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mclicks.h"
+#include "mc_internal.h"
+#include "mc_kernels.h"
+#include "mc_model.h"
+
+namespace mc {
+
+struct EvalArgs {
+    // Session-major layout.
+    const int32_t* pidx;     // [n][10]
+    const uint8_t* clicks;   // [n][10]
+    // Rank-major layout.
+    const int32_t* rpidx;    // [10][n]
+    const uint16_t* mask;    // [n]
+    const int32_t* perm;     // [n]: caller's index of the session at position i, or null
+    const uint8_t* len;      // [n] or null (10 ranks)
+    int64_t n;
+    int64_t n_qd;            // pair indices outside [0, n_qd) read as unseen pairs
+    const float* val0;       // attractiveness values
+    const float* val1;       // satisfaction values (DBN)
+    const float* sh;         // shared values
+    float* logp;             // [n][10] or null
+    float* full;             // [n][10] or null
+    float* llh;              // [n] or null
+    float* prob;             // [n][10] or null: get_serp_probability
+    double* partial;         // [gridDim.x][21]: llh, ppl[10], count[10]
+};
+
+// Value of pair p, or PARAM_DEF_NUM / PARAM_DEF_DENOM = 1/2 for a pair unseen
+// in training (p = -1).  Indices the host has not checked (device-resident
+// inputs) are bounded here: any p outside [0, n_qd) reads as unseen.
+__device__ __forceinline__ float pv_or_default(const float* v, int p, int64_t n_qd) {
+    return (uint64_t)(uint32_t)p < (uint64_t)n_qd ? v[p] : 0.5f;
+}
+
+// One session's pair indices and clicks, session-major: the ten indices as
+// five 8-byte loads (rows are 40 B, so 8-byte aligned), the click bytes as
+// five 2-byte loads.
+__device__ __forceinline__ void load_session_major(const EvalArgs& E, int64_t i, int (&p)[kL], bool (&c)[kL]) {
+    const int2* p2 = reinterpret_cast<const int2*>(E.pidx + i * kL);
+#pragma unroll
+    for (int u = 0; u < kL / 2; u++) {
+        const int2 v = p2[u];
+        p[2 * u] = v.x;
+        p[2 * u + 1] = v.y;
+    }
+    const uint16_t* c2 = reinterpret_cast<const uint16_t*>(E.clicks + i * kL);
+#pragma unroll
+    for (int u = 0; u < kL / 2; u++) {
+        const uint16_t v = c2[u];
+        c[2 * u] = (v & 0xffu) == 1;
+        c[2 * u + 1] = (v >> 8) == 1;
+    }
+}
+
+// Rank-major: rank r of every session is one contiguous row, so a
+// wavefront's 64 lanes read 256 contiguous bytes per rank; ranks past the
+// length are not read.
+__device__ __forceinline__ void load_rank_major(const EvalArgs& E, int64_t i, int L, int (&p)[kL], bool (&c)[kL]) {
+    const uint32_t m = E.mask[i];
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        p[r] = r < L ? E.rpidx[r * E.n + i] : -1;
+        c[r] = (m >> r) & 1u;
+    }
+}
+
+// The per-session arithmetic, the reference's operation for operation in
+// FP32: log conditional click probabilities lp, full click probabilities
+// fp and the SERP click probabilities sp of the first L ranks.
+template <int MODEL>
+__device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const int (&p)[kL], const bool (&c)[kL],
+                                             float (&lp)[kL], float (&fp)[kL], float (&sp)[kL]) {
+    float a[kL], sv[kL];
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        a[r] = r < L ? pv_or_default(E.val0, p[r], E.n_qd) : 0.5f;
+        sv[r] = (MODEL == kDBN && r < L) ? pv_or_default(E.val1, p[r], E.n_qd) : 0.5f;
+    }
+    if (MODEL == kPBM) {
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * E.sh[r];
+                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
+                fp[r] = c[r] ? ae : 1 - ae;
+                sp[r] = ae;
+            }
+        }
+    } else if (MODEL == kUBM) {
+        int prev[kL];
+        int last = kL - 1;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            prev[r] = last;
+            if (r < L && c[r]) last = r;
+        }
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * E.sh[r * (r + 1) / 2 + prev[r]];
+                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
+                sp[r] = ae;
+            }
+        }
+        // P(C_r = 1) summed over the rank of the previous click (-1: none).
+        float tf[kL];
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float cp = 0.f;
+#pragma unroll
+                for (int rp = -1; rp < r; rp++) {
+                    float nb = 1.f;
+                    const int crp = rp == -1 ? 9 : rp;
+#pragma unroll
+                    for (int rb = rp + 1; rb < r; rb++) nb *= 1 - a[rb] * E.sh[rb * (rb + 1) / 2 + crp];
+                    float temp = nb * (a[r] * E.sh[r * (r + 1) / 2 + crp]);
+                    cp += rp >= 0 ? tf[rp] * temp : temp;
+                }
+                tf[r] = cp;
+                fp[r] = c[r] ? cp : 1 - cp;
+            } else {
+                tf[r] = 0.f;
+            }
+        }
+    } else if (MODEL == kDBN) {
+        const float g = E.sh[0];
+        float ex = 1.f;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float cp;
+                if (c[r]) { cp = a[r] * ex; ex = g * (1 - sv[r]); }
+                else { cp = 1 - a[r] * ex; ex *= g * (1 - a[r]) / cp; }
+                lp[r] = logf(cp);
+                sp[r] = cp;
+            }
+        }
+        ex = 1.f;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * ex;
+                fp[r] = c[r] ? ae : 1 - ae;
+                ex *= g * (1 - a[r]) + g * a[r] * (1 - sv[r]);
+            }
+        }
+    } else {
+        const float t1 = E.sh[0], t2 = E.sh[1], t3 = E.sh[2];
+        float ex = 1.f;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float cp;
+                if (c[r]) { cp = a[r] * ex; ex = t2 * (1 - a[r]) + t3 * a[r]; }
+                else { cp = 1 - a[r] * ex; ex *= t1 * (1 - a[r]) / cp; }
+                lp[r] = logf(cp);
+                sp[r] = cp;
+            }
+        }
+        ex = 1.f;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                float ae = a[r] * ex;
+                fp[r] = c[r] ? ae : 1 - ae;
+                ex *= (1 - a[r]) * t1 + a[r] * ((1 - a[r]) * t2 + a[r] * t3);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One session per lane, grid-stride: in every pass a wavefront takes 64
+// consecutive sessions.  Per-block FP64 partials of the 21 sums.
+template <int MODEL, bool RANK_MAJOR>
+__global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
+    __shared__ double red[21][4];
+    const int tid = threadIdx.x;
+    double v[21];
+#pragma unroll
+    for (int k = 0; k < 21; k++) v[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < E.n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int L = E.len ? min(max((int)E.len[i], 1), kL) : kL;
+        int p[kL];
+        bool c[kL];
+        if (RANK_MAJOR) load_rank_major(E, i, L, p, c);
+        else load_session_major(E, i, p, c);
+        float lp[kL], fp[kL], sp[kL];
+        eval_session<MODEL>(E, L, p, c, lp, fp, sp);
+        float acc = 0.f;
+#pragma unroll
+        for (int r = 0; r < kL; r++)
+            if (r < L) acc = acc + lp[r];
+        const float s = acc / L;                 // evaluation.cpp:36
+        v[0] += (double)s;
+        const int64_t o = (RANK_MAJOR && E.perm) ? (int64_t)E.perm[i] : i;
+        if (E.llh) E.llh[o] = s;
+#pragma unroll
+        for (int r = 0; r < kL; r++) {
+            if (r < L) {
+                v[1 + r] += (double)log2f(fp[r]);   // evaluation.cpp:62
+                v[11 + r] += 1.0;
+            }
+            if (E.logp) E.logp[o * kL + r] = r < L ? lp[r] : 0.f;
+            if (E.full) E.full[o * kL + r] = r < L ? fp[r] : 0.f;
+            if (E.prob) E.prob[o * kL + r] = r < L ? sp[r] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 21; k++) {
+        double w = wave_sum_d(v[k]);
+        if ((tid & 63) == 0) red[k][tid >> 6] = w;
+    }
+    __syncthreads();
+    if (tid < 21) {
+        double w = 0.0;
+        for (int j = 0; j < (int)(blockDim.x >> 6); j++) w += red[tid][j];
+        E.partial[(int64_t)blockIdx.x * 21 + tid] = w;
+    }
+}
+
+// Sums block partials: one block per output, each thread a fixed strided
+// subset, then a fixed tree; the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) eval_reduce_kernel(const double* partial, int n_blocks, double* out) {
+    __shared__ double red[256];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    double w = 0.0;
+    for (int b = tid; b < n_blocks; b += 256) w += partial[(int64_t)b * 21 + k];
+    red[tid] = w;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) red[tid] += red[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) out[k] = red[0];
+}
+
+// Evaluation blocks per launch: at most this many (grid-stride beyond).
+constexpr int64_t kEvalMaxBlocks = 4096;
+
+int64_t eval_blocks(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, kEvalMaxBlocks)); }
+
+template <bool RANK_MAJOR>
+void launch_eval(int type, const EvalArgs& E, int64_t nb, hipStream_t s) {
+    switch (type) {
+        case MC_PBM: hipLaunchKernelGGL((eval_kernel<kPBM, RANK_MAJOR>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+        case MC_UBM: hipLaunchKernelGGL((eval_kernel<kUBM, RANK_MAJOR>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+        case MC_DBN: hipLaunchKernelGGL((eval_kernel<kDBN, RANK_MAJOR>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+        default: hipLaunchKernelGGL((eval_kernel<kCCM, RANK_MAJOR>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+    }
+}
+
+EvalArgs model_eval_args(const mc_model* m) {
+    EvalArgs E;
+    std::memset(&E, 0, sizeof(E));
+    E.val0 = m->d_val[m->cur][0];
+    E.val1 = m->T > 1 ? m->d_val[m->cur][1] : m->d_val[m->cur][0];
+    E.sh = m->d_sh_val[m->cur];
+    E.n_qd = m->n_qd;
+    return E;
+}
+
+// Checks the caller's session arrays (session-major, host memory).
+mc_status check_sessions(const mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks,
+                         const uint8_t* len) {
+    for (int64_t i = 0; i < n; i++) {
+        const int l = len ? len[i] : kMaxSerp;
+        MC_REQUIRE(l >= 1 && l <= kMaxSerp, "len must be in [1, 10] (session " + std::to_string(i) + ")");
+        for (int r = 0; r < l; r++) {
+            const int32_t p = pidx[i * kMaxSerp + r];
+            MC_REQUIRE(p >= -1 && p < m->n_qd, "test pidx out of [-1, n_qd) (session " + std::to_string(i) + ")");
+            MC_REQUIRE(clicks[i * kMaxSerp + r] <= 1, "clicks must be 0 or 1");
+        }
+    }
+    return MC_OK;
+}
+
+void unpack_totals(const double (&out)[21], double* llh_sum, double* ppl_log2_sum, int64_t* rank_count) {
+    if (llh_sum) *llh_sum = out[0];
+    for (int r = 0; r < kMaxSerp; r++) {
+        if (ppl_log2_sum) ppl_log2_sum[r] = out[1 + r];
+        if (rank_count) rank_count[r] = (int64_t)out[11 + r];
+    }
+}
+
+// Session-major evaluation of caller arrays in host memory, with temporary
+// device buffers for this call.
+mc_status evaluate_host_arrays(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                               float* logp, float* full, float* llh, float* prob, double (&out)[21]) {
+    MC_HIP(hipSetDevice(m->device));
+    const int64_t nb = eval_blocks(n);
+    std::vector<void*> tmp;
+    auto talloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+        tmp.push_back(p);
+        return p;
+    };
+    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
+    int32_t* d_p = (int32_t*)talloc((size_t)n * kMaxSerp * 4);
+    uint8_t* d_c = (uint8_t*)talloc((size_t)n * kMaxSerp);
+    uint8_t* d_l = len ? (uint8_t*)talloc((size_t)n) : nullptr;
+    float* d_lp = logp ? (float*)talloc((size_t)n * kMaxSerp * 4) : nullptr;
+    float* d_fp = full ? (float*)talloc((size_t)n * kMaxSerp * 4) : nullptr;
+    float* d_llh = llh ? (float*)talloc((size_t)n * 4) : nullptr;
+    float* d_pr = prob ? (float*)talloc((size_t)n * kMaxSerp * 4) : nullptr;
+    double* d_part = (double*)talloc((size_t)nb * 21 * 8);
+    double* d_out = (double*)talloc(21 * 8);
+    if (!d_p || !d_c || (len && !d_l) || (logp && !d_lp) || (full && !d_fp) || (llh && !d_llh) || (prob && !d_pr) ||
+        !d_part || !d_out) {
+        cleanup();
+        set_error("device allocation failed in the evaluation");
+        return MC_ERR_OOM;
+    }
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess && x != hipSuccess) e = x; };
+    chk(hipMemcpyAsync(d_p, pidx, (size_t)n * kMaxSerp * 4, hipMemcpyHostToDevice, m->stream));
+    chk(hipMemcpyAsync(d_c, clicks, (size_t)n * kMaxSerp, hipMemcpyHostToDevice, m->stream));
+    if (len) chk(hipMemcpyAsync(d_l, len, (size_t)n, hipMemcpyHostToDevice, m->stream));
+    EvalArgs E = model_eval_args(m);
+    E.pidx = d_p;
+    E.clicks = d_c;
+    E.len = d_l;
+    E.n = n;
+    E.logp = d_lp;
+    E.full = d_fp;
+    E.llh = d_llh;
+    E.prob = d_pr;
+    E.partial = d_part;
+    launch_eval<false>(m->type, E, nb, m->stream);
+    chk(hipGetLastError());
+    hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, d_part, (int)nb, d_out);
+    chk(hipGetLastError());
+    chk(hipMemcpyAsync(out, d_out, 21 * 8, hipMemcpyDeviceToHost, m->stream));
+    if (logp) chk(hipMemcpyAsync(logp, d_lp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
+    if (full) chk(hipMemcpyAsync(full, d_fp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
+    if (llh) chk(hipMemcpyAsync(llh, d_llh, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream));
+    if (prob) chk(hipMemcpyAsync(prob, d_pr, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
+    chk(hipStreamSynchronize(m->stream));
+    cleanup();
+    if (e != hipSuccess) {
+        set_error(std::string("evaluation: ") + hipGetErrorString(e));
+        return MC_ERR_HIP;
+    }
+    return MC_OK;
+}
+
+}  // namespace mc
+
+using namespace mc;
+
+extern "C" {
+
+mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                      float* logp, float* full, float* llh, double* llh_sum, double* ppl_log2_sum,
+                      int64_t* rank_count) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(n >= 1 && pidx && clicks, "invalid argument");
+    MC_TRY(check_sessions(m, n, pidx, clicks, len));
+    double out[21];
+    MC_TRY(evaluate_host_arrays(m, n, pidx, clicks, len, logp, full, llh, nullptr, out));
+    unpack_totals(out, llh_sum, ppl_log2_sum, rank_count);
+    return MC_OK;
+}
+
+mc_status mc_serp_probability(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                              float* prob) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(n >= 1 && pidx && clicks && prob, "invalid argument");
+    MC_TRY(check_sessions(m, n, pidx, clicks, len));
+    double out[21];
+    return evaluate_host_arrays(m, n, pidx, clicks, len, nullptr, nullptr, nullptr, prob, out);
+}
+
+mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(n >= 1 && pidx && clicks, "invalid argument");
+    MC_REQUIRE(n <= INT32_MAX, "too many test sessions");
+    MC_TRY(check_sessions(m, n, pidx, clicks, len));
+    MC_HIP(hipSetDevice(m->device));
+    // Order: by the smallest valid pair index of each session.  Pair indices
+    // are assigned per query in contiguous ranges, so this groups the
+    // sessions of one query together; sessions without a valid index go
+    // last.  Counting sort: O(n + n_qd), stable.
+    const int64_t n_qd = m->n_qd;
+    std::vector<int32_t> key((size_t)n);
+    std::vector<int64_t> start((size_t)n_qd + 2, 0);
+    for (int64_t i = 0; i < n; i++) {
+        const int l = len ? len[i] : kMaxSerp;
+        int32_t k = (int32_t)n_qd;
+        for (int r = 0; r < l; r++) {
+            const int32_t p = pidx[i * kMaxSerp + r];
+            if (p >= 0 && p < k) k = p;
+        }
+        key[(size_t)i] = k;
+        start[(size_t)k + 1]++;
+    }
+    for (int64_t k = 0; k <= n_qd; k++) start[(size_t)k + 1] += start[(size_t)k];
+    std::vector<int32_t> perm((size_t)n);
+    for (int64_t i = 0; i < n; i++) perm[(size_t)start[(size_t)key[(size_t)i]]++] = (int32_t)i;
+    std::vector<int32_t> h_p((size_t)n * kMaxSerp);
+    std::vector<uint16_t> h_m((size_t)n);
+    std::vector<uint8_t> h_l(len ? (size_t)n : 0);
+    for (int64_t j = 0; j < n; j++) {
+        const int64_t i = perm[(size_t)j];
+        const int l = len ? len[i] : kMaxSerp;
+        uint16_t mk = 0;
+        for (int r = 0; r < kMaxSerp; r++) {
+            h_p[(size_t)r * n + j] = r < l ? pidx[i * kMaxSerp + r] : -1;
+            if (r < l && clicks[i * kMaxSerp + r]) mk |= (uint16_t)(1u << r);
+        }
+        h_m[(size_t)j] = mk;
+        if (len) h_l[(size_t)j] = (uint8_t)l;
+    }
+    // Buffers of an earlier test set are replaced.
+    for (void* p : {(void*)m->d_test_pidx, (void*)m->d_test_mask, (void*)m->d_test_len, (void*)m->d_test_perm,
+                    (void*)m->d_test_part}) {
+        if (!p) continue;
+        auto it = std::find(m->allocations.begin(), m->allocations.end(), p);
+        if (it != m->allocations.end()) m->allocations.erase(it);
+        (void)hipFree(p);
+    }
+    m->mem_bytes -= m->test_bytes;
+    m->test_bytes = 0;
+    m->d_test_pidx = nullptr;
+    m->d_test_mask = nullptr;
+    m->d_test_len = nullptr;
+    m->d_test_perm = nullptr;
+    m->d_test_part = nullptr;
+    m->n_test = 0;
+    const size_t before = m->mem_bytes;
+    const int64_t nb = eval_blocks(n);
+    MC_TRY(dalloc(m, &m->d_test_pidx, h_p.size() * 4));
+    MC_TRY(dalloc(m, &m->d_test_mask, h_m.size() * 2));
+    if (len) MC_TRY(dalloc(m, &m->d_test_len, h_l.size()));
+    MC_TRY(dalloc(m, &m->d_test_perm, perm.size() * 4));
+    MC_TRY(dalloc(m, &m->d_test_part, ((size_t)nb * 21 + 21) * 8));
+    m->test_bytes = m->mem_bytes - before;
+    MC_HIP(hipMemcpyAsync(m->d_test_pidx, h_p.data(), h_p.size() * 4, hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_test_mask, h_m.data(), h_m.size() * 2, hipMemcpyHostToDevice, m->stream));
+    if (len) MC_HIP(hipMemcpyAsync(m->d_test_len, h_l.data(), h_l.size(), hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemcpyAsync(m->d_test_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipStreamSynchronize(m->stream));
+    m->n_test = n;
+    return MC_OK;
+}
+
+mc_status mc_evaluate_test_set(mc_model* m, float* logp, float* full, float* llh, double* llh_sum,
+                               double* ppl_log2_sum, int64_t* rank_count, double* kernel_ms) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(m->n_test > 0, "no test set: call mc_set_test_set first");
+    MC_HIP(hipSetDevice(m->device));
+    const int64_t n = m->n_test;
+    const int64_t nb = eval_blocks(n);
+    std::vector<void*> tmp;
+    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
+    auto talloc = [&](size_t bytes) -> float* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+        tmp.push_back(p);
+        return (float*)p;
+    };
+    float* d_lp = logp ? talloc((size_t)n * kMaxSerp * 4) : nullptr;
+    float* d_fp = full ? talloc((size_t)n * kMaxSerp * 4) : nullptr;
+    float* d_llh = llh ? talloc((size_t)n * 4) : nullptr;
+    if ((logp && !d_lp) || (full && !d_fp) || (llh && !d_llh)) {
+        cleanup();
+        set_error("device allocation failed in mc_evaluate_test_set");
+        return MC_ERR_OOM;
+    }
+    EvalArgs E = model_eval_args(m);
+    E.rpidx = m->d_test_pidx;
+    E.mask = m->d_test_mask;
+    E.perm = m->d_test_perm;
+    E.len = m->d_test_len;
+    E.n = n;
+    E.logp = d_lp;
+    E.full = d_fp;
+    E.llh = d_llh;
+    E.partial = m->d_test_part;
+    double* d_out = m->d_test_part + nb * 21;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (e == hipSuccess && x != hipSuccess) e = x; };
+    if (kernel_ms) {
+        chk(hipEventCreate(&ev[0]));
+        chk(hipEventCreate(&ev[1]));
+        chk(hipEventRecord(ev[0], m->stream));
+    }
+    launch_eval<true>(m->type, E, nb, m->stream);
+    chk(hipGetLastError());
+    if (kernel_ms) chk(hipEventRecord(ev[1], m->stream));
+    hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, (const double*)m->d_test_part, (int)nb,
+                       d_out);
+    chk(hipGetLastError());
+    double out[21];
+    chk(hipMemcpyAsync(out, d_out, 21 * 8, hipMemcpyDeviceToHost, m->stream));
+    if (logp) chk(hipMemcpyAsync(logp, d_lp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
+    if (full) chk(hipMemcpyAsync(full, d_fp, (size_t)n * kMaxSerp * 4, hipMemcpyDeviceToHost, m->stream));
+    if (llh) chk(hipMemcpyAsync(llh, d_llh, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream));
+    chk(hipStreamSynchronize(m->stream));
+    if (kernel_ms && e == hipSuccess) {
+        float ms = 0.f;
+        chk(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        *kernel_ms = ms;
+    }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    cleanup();
+    if (e != hipSuccess) {
+        set_error(std::string("mc_evaluate_test_set: ") + hipGetErrorString(e));
+        return MC_ERR_HIP;
+    }
+    unpack_totals(out, llh_sum, ppl_log2_sum, rank_count);
+    return MC_OK;
+}
+
+}  // extern "C"

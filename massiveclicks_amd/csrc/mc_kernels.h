@@ -151,14 +151,51 @@ __device__ __forceinline__ int last_click(uint32_t m) {
 // t for the entry p shown at rank r of the current session; add_wide takes
 // any non-negative x (the CCM's last-click attractiveness term can exceed 1;
 // the last-click terms, once per session, always take it).
+#ifndef MC_ACC_SPLIT
+#define MC_ACC_SPLIT 0
+#endif
+#ifndef MC_ACC_REPL
+#define MC_ACC_REPL 0
+#endif
+
+// LDS accumulator window of a work item: `wstride` slots per per-qd table.
+// Entry p of a window of W entries owns 2^rsh consecutive slots (copies),
+// 2^rsh <= wstride / W: lane l adds into copy l mod 2^rsh, which spreads the
+// atomics of lanes that hit one entry over several addresses, and the flush
+// adds the copies.
+template <int T>
+struct AccWindow {
+    unsigned long long* acc;
+    int wstride;
+    int rsh;
+    __device__ __forceinline__ int slot(int t, int q) const {   // q = (p << rsh) + copy
+        return MC_ACC_SPLIT ? t * wstride + q : q * T + t;
+    }
+    __device__ __forceinline__ unsigned long long sum(int t, int p) const {
+        if (rsh == 0) return acc[slot(t, p)];
+        unsigned long long v = 0;
+        for (int k = 0; k < (1 << rsh); k++) v += acc[slot(t, (p << rsh) + k)];
+        return v;
+    }
+};
+
+// Copies per entry of a window of W entries in wstride slots (a power of
+// two, at most 64).
+__device__ __forceinline__ int acc_copies_log2(int W, int wstride) {
+    if (!MC_ACC_REPL || W <= 0) return 0;
+    const int q = wstride / W;
+    return q <= 1 ? 0 : min(6, 31 - __builtin_clz((unsigned)q));
+}
+
 template <int T>
 struct EmitLds {            // fixed-point LDS window, relative indices
-    unsigned long long* acc;   // [width][T]
+    AccWindow<T> w;
+    int copy;                  // this lane's copy
     __device__ __forceinline__ void add(int t, int, int p, float x) {
-        atomicAdd(&acc[p * T + t], to_fix_unit(x));
+        atomicAdd(&w.acc[w.slot(t, (p << w.rsh) + copy)], to_fix_unit(x));
     }
     __device__ __forceinline__ void add_wide(int t, int, int p, float x) {
-        atomicAdd(&acc[p * T + t], (unsigned long long)to_fix(x));
+        atomicAdd(&w.acc[w.slot(t, (p << w.rsh) + copy)], (unsigned long long)to_fix(x));
     }
 };
 struct EmitGlobalFix {      // fixed-point global (item without a window)

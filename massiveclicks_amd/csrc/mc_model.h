@@ -116,6 +116,14 @@ struct mc_model {
     hipGraphExec_t adam_graph = nullptr;       // one epoch (every batch once) of Adam steps
     int adam_graph_first = -1;
     bool value_only = false;                   // tables hold values only (numerators stale)
+    // ---- resident held-out set (mc_eval.hip) --------------------------------
+    int32_t* d_test_pidx = nullptr;            // [10][n_test] pair indices (-1 past the length / unseen)
+    uint16_t* d_test_mask = nullptr;           // [n_test] click bits
+    uint8_t* d_test_len = nullptr;             // [n_test] or null (10 ranks)
+    int32_t* d_test_perm = nullptr;            // [n_test] caller's index of each stored session
+    double* d_test_part = nullptr;             // evaluation partials and totals
+    int64_t n_test = 0;
+    size_t test_bytes = 0;
     // graph cache, one slot per starting buffer parity
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_iters[2] = {0, 0};

@@ -108,6 +108,22 @@ def _sessions_arrays(sessions: SessionSet, what: str):
     return p, c, ln
 
 
+def _eval_summary(n, llh_sum, ppl, cnt, per_session):
+    """LL and perplexity from the FP64 totals (parallel_em.cu:697-724)."""
+    ppl_rank = [2.0 ** (-ppl[r] / cnt[r]) if cnt[r] else float("nan") for r in range(MAX_SERP)]
+    out = {
+        "llh_sum": llh_sum,
+        "log_likelihood": llh_sum / n,                 # parallel_em.cu:704
+        "ppl_log2_sum": ppl,
+        "rank_count": cnt,
+        "perplexity_rank": ppl_rank,                  # parallel_em.cu:719-721
+        "perplexity": float(np.mean(ppl_rank)),        # parallel_em.cu:723
+    }
+    if per_session is not None:
+        out.update(logp=per_session[0], full=per_session[1], llh=per_session[2])
+    return out
+
+
 class ClickModel:
     """One click model on one GPU (ClickModel_Hst, base.cuh:30-53)."""
 
@@ -276,17 +292,33 @@ class ClickModel:
         check(_lib.lib().mc_evaluate(self._h, n, ptr(p, i32p), ptr(c, u8p), ptr(ln, u8p), ptr(logp, f32p),
                                      ptr(full, f32p), ptr(llh, f32p), C.byref(llh_sum), ptr(ppl, f64p),
                                      ptr(cnt, i64p)))
-        ppl_rank = [2.0 ** (-ppl[r] / cnt[r]) if cnt[r] else float("nan") for r in range(MAX_SERP)]
-        out = {
-            "llh_sum": llh_sum.value,
-            "log_likelihood": llh_sum.value / n,          # parallel_em.cu:704
-            "ppl_log2_sum": ppl,
-            "rank_count": cnt,
-            "perplexity_rank": ppl_rank,                  # parallel_em.cu:719-721
-            "perplexity": float(np.mean(ppl_rank)),        # parallel_em.cu:723
-        }
-        if per_session:
-            out.update(logp=logp, full=full, llh=llh)
+        return _eval_summary(n, llh_sum.value, ppl, cnt, (logp, full, llh) if per_session else None)
+
+    def set_test_set(self, test: SessionSet):
+        """Keeps the held-out set on the device (mc_set_test_set) for
+        evaluate_test_set."""
+        p, c, ln = _sessions_arrays(test, "set_test_set")
+        check(_lib.lib().mc_set_test_set(self._h, len(test), ptr(p, i32p), ptr(c, u8p), ptr(ln, u8p)))
+        self._n_test = len(test)
+
+    def evaluate_test_set(self, per_session: bool = False, timed: bool = False):
+        """evaluate() on the resident held-out set; per-session outputs in
+        the order the set was given.  timed adds the evaluation kernel's
+        duration in ms ("kernel_ms")."""
+        n = getattr(self, "_n_test", 0)
+        logp = np.empty((n, MAX_SERP), np.float32) if per_session else None
+        full = np.empty((n, MAX_SERP), np.float32) if per_session else None
+        llh = np.empty(n, np.float32) if per_session else None
+        llh_sum = C.c_double()
+        ppl = np.empty(MAX_SERP, np.float64)
+        cnt = np.empty(MAX_SERP, np.int64)
+        ms = C.c_double()
+        check(_lib.lib().mc_evaluate_test_set(self._h, ptr(logp, f32p), ptr(full, f32p), ptr(llh, f32p),
+                                              C.byref(llh_sum), ptr(ppl, f64p), ptr(cnt, i64p),
+                                              C.byref(ms) if timed else None))
+        out = _eval_summary(n, llh_sum.value, ppl, cnt, (logp, full, llh) if per_session else None)
+        if timed:
+            out["kernel_ms"] = ms.value
         return out
 
     def get_log_conditional_click_probs(self, test: SessionSet):
@@ -294,6 +326,14 @@ class ClickModel:
 
     def get_full_click_probs(self, test: SessionSet):
         return self.evaluate(test, per_session=True)["full"]
+
+    def get_serp_probability(self, test: SessionSet):
+        """get_serp_probability (base.cuh:50) of every session: [n, 10]."""
+        p, c, ln = _sessions_arrays(test, "get_serp_probability")
+        out = np.empty((len(test), MAX_SERP), np.float32)
+        check(_lib.lib().mc_serp_probability(self._h, len(test), ptr(p, i32p), ptr(c, u8p), ptr(ln, u8p),
+                                             ptr(out, f32p)))
+        return out
 
     # -- bookkeeping -------------------------------------------------------------
     def get_memory_usage(self) -> int:

@@ -57,6 +57,7 @@ def lib():
             "orc_set_table": (None, [vp, C.c_int, f32p, f32p]),
             "orc_get_tmp": (None, [vp, C.c_int, f32p, f32p]),
             "orc_eval": (None, [vp, C.c_int64, i32p, u8p, u8p, f32p, f32p, f32p, f32p, f64p, f32p, f64p, i64p]),
+            "orc_serp_probability": (None, [vp, C.c_int64, i32p, u8p, u8p, f32p]),
             "orc_model_free": (None, [vp]),
         }
         for name, (res, args) in sigs.items():
@@ -165,6 +166,14 @@ class OracleModel:
         num, den = np.empty(n, np.float32), np.empty(n, np.float32)
         lib().orc_get_tmp(self.h, t, _p(num, f32p), _p(den, f32p))
         return num, den
+
+    def serp_probability(self, pidx, clicks, length=None):
+        p = np.ascontiguousarray(pidx, np.int32)
+        c = np.ascontiguousarray(clicks, np.uint8)
+        ln = None if length is None else np.ascontiguousarray(length, np.uint8)
+        out = np.empty((len(p), MAX_SERP), np.float32)
+        lib().orc_serp_probability(self.h, len(p), _p(p, i32p), _p(c, u8p), _p(ln, u8p), _p(out, f32p))
+        return out
 
     def evaluate(self, pidx, clicks, length=None):
         p = np.ascontiguousarray(pidx, np.int32)

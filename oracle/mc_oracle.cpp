@@ -739,6 +739,57 @@ void eval_session(const Model& m, const int32_t* p, const uint8_t* c, int L, flo
     }
 }
 
+// get_serp_probability (pbm.cu:351-365, ubm.cu:345-362, dbn.cu:572-600,
+// ccm.cu:566-593): PBM/UBM give a * e whatever the click; DBN/CCM give the
+// conditional probability of the observed outcome along the same chain as
+// get_log_conditional_click_probs, without the log.
+void serp_probability(const Model& m, const int32_t* p, const uint8_t* c, int L, float* prob) {
+    switch (m.type) {
+        case PBM:
+            for (int r = 0; r < L; r++) prob[r] = pval(m.tab[0], p[r]) * m.tab[1][r].value();
+            break;
+        case UBM: {
+            int prev[kMaxSerp] = {0};
+            prev_clicked_rank(c, L, prev);
+            for (int r = 0; r < L; r++) prob[r] = pval(m.tab[0], p[r]) * m.tab[1][r * (r + 1) / 2 + prev[r]].value();
+            break;
+        }
+        case DBN: {
+            float ex{1.f}, cp;
+            float g = m.tab[2][0].value();
+            for (int r = 0; r < L; r++) {
+                float a = pval(m.tab[0], p[r]);
+                float s = pval(m.tab[1], p[r]);
+                if (c[r] == 1) {
+                    cp = a * ex;
+                    ex = g * (1 - s);
+                } else {
+                    cp = 1 - a * ex;
+                    ex *= g * (1 - a) / cp;
+                }
+                prob[r] = cp;
+            }
+            break;
+        }
+        case CCM: {
+            float ex{1.f}, cp;
+            float t1 = m.tab[1][0].value(), t2 = m.tab[1][1].value(), t3 = m.tab[1][2].value();
+            for (int r = 0; r < L; r++) {
+                float a = pval(m.tab[0], p[r]);
+                if (c[r] == 1) {
+                    cp = a * ex;
+                    ex = t2 * (1 - a) + t3 * a;
+                } else {
+                    cp = 1 - a * ex;
+                    ex *= t1 * (1 - a) / cp;
+                }
+                prob[r] = cp;
+            }
+            break;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Dataset: parse (dataset.cpp:224-305, search.cu:93-122), split
 // (dataset.cpp:385-396), sort (dataset.cpp:17-21), qd indices
@@ -1007,6 +1058,17 @@ void orc_eval(const orc_model* h, int64_t n, const int32_t* pidx, const uint8_t*
         ppl_rank_f32[r] = pr32[r];
         ppl_rank_f64[r] = pr64[r];
         rank_count[r] = cnt[r];
+    }
+}
+
+// get_serp_probability per session; prob is n x 10 (0 past the length).
+void orc_serp_probability(const orc_model* h, int64_t n, const int32_t* pidx, const uint8_t* clicks,
+                          const uint8_t* len, float* prob) {
+    for (int64_t i = 0; i < n; i++) {
+        int L = len ? len[i] : kMaxSerp;
+        float pr[kMaxSerp];
+        serp_probability(h->m, pidx + i * kMaxSerp, clicks + i * kMaxSerp, L, pr);
+        for (int r = 0; r < kMaxSerp; r++) prob[i * kMaxSerp + r] = r < L ? pr[r] : 0.f;
     }
 }
 

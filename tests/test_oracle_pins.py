@@ -261,6 +261,48 @@ def test_ubm_full_click_probs_over_all_click_masks():
 
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_serp_probability_matches_enumeration(model):
+    """get_serp_probability (pbm.cu:351-365, ubm.cu:345-362, dbn.cu:572-600,
+    ccm.cu:566-593) on every click pattern, against enumeration of hidden
+    states: for the PBM and UBM it is P(C_r = 1 | the earlier clicks)
+    whatever rank r's own click (ten ranks, 1024 patterns); for the DBN and
+    CCM it is P(C_r = c_r | the earlier clicks), the observed outcome's
+    conditional probability (five ranks, 32 patterns)."""
+    from bruteforce import cascade_joint
+    rng = np.random.default_rng(60 + model)
+    if model in (PBM, UBM):
+        L = 10
+        a = rng.uniform(0.05, 0.9, L).astype(np.float32)
+        e = rng.uniform(0.05, 0.95, 10 if model == PBM else 55).astype(np.float32)
+        ef = e.astype(np.float64)
+        if model == PBM:
+            C, p = independent_joint(a.astype(np.float64), ef)
+        else:
+            C, p = ubm_joint(a.astype(np.float64), lambda r, prev: ef[ubm_expected_slot(r, prev)])
+        cond = conditional_from_patterns(pattern_probs(C, p), L)
+        masks = np.arange(1 << L)
+        want = np.stack([cond[masks | (1 << r), r] for r in range(L)], axis=1)
+        tables = {0: a, 1: e}
+    else:
+        L = 5
+        a = rng.uniform(0.05, 0.9, L).astype(np.float32)
+        s = rng.uniform(0.05, 0.9, L).astype(np.float32)
+        g = np.float32(rng.uniform(0.3, 0.95))
+        taus = rng.uniform(0.2, 0.95, 3).astype(np.float32)
+        if model == DBN:
+            C, _, _, _, p = cascade_joint("dbn", a=a.astype(np.float64), s=s.astype(np.float64), g=float(g))
+            tables = {0: a, 1: s, 2: [g]}
+        else:
+            C, _, _, _, p = cascade_joint("ccm", a=a.astype(np.float64), taus=taus.astype(np.float64))
+            tables = {0: a, 1: taus}
+        want = conditional_from_patterns(pattern_probs(C, p), L)
+    om, pidx, clicks, length = all_masks_model(model, L, tables)
+    got = om.serp_probability(pidx, clicks, length)
+    np.testing.assert_allclose(got[:, :L], want, rtol=1e-5)
+    assert np.all(got[:, L:] == 0)
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
 def test_session_mean_and_totals_follow_evaluation(model):
     """LogLikelihood::evaluate and Perplexity::evaluate
     (evaluation.cpp:25-40, 49-66): a session's log-likelihood is the float
