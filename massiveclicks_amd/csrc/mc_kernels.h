@@ -144,33 +144,19 @@ __device__ __forceinline__ int last_click(uint32_t m) {
     return m ? 31 - __builtin_clz(m) : kL;
 }
 
-// --------------------------------------------------------------------------
-// Emitters: where a per-qd expected count goes.
-// --------------------------------------------------------------------------
-// add(t, r, p, x): expected count x, a posterior in [0, 1], of per-qd table
-// t for the entry p shown at rank r of the current session; add_wide takes
-// any non-negative x (the CCM's last-click attractiveness term can exceed 1;
-// the last-click terms, once per session, always take it).
-#ifndef MC_ACC_SPLIT
-#define MC_ACC_SPLIT 0
-#endif
-#ifndef MC_ACC_REPL
-#define MC_ACC_REPL 0
-#endif
-
-// LDS accumulator window of a work item: `wstride` slots per per-qd table.
-// Entry p of a window of W entries owns 2^rsh consecutive slots (copies),
-// 2^rsh <= wstride / W: lane l adds into copy l mod 2^rsh, which spreads the
-// atomics of lanes that hit one entry over several addresses, and the flush
-// adds the copies.
+// LDS accumulator window of a work item: `wstride` slots per per-qd table,
+// table t's slots contiguous (a float2-wide entry stride would halve the
+// banks that one table's 8-byte atomics spread over).  Entry p of a window of
+// W entries owns 2^rsh consecutive slots (copies), 2^rsh <= wstride / W:
+// lane l adds into copy l mod 2^rsh, which spreads the atomics of lanes that
+// hit one entry (the narrow windows of the heaviest queries) over several
+// addresses, and the flush adds the copies.
 template <int T>
 struct AccWindow {
     unsigned long long* acc;
     int wstride;
     int rsh;
-    __device__ __forceinline__ int slot(int t, int q) const {   // q = (p << rsh) + copy
-        return MC_ACC_SPLIT ? t * wstride + q : q * T + t;
-    }
+    __device__ __forceinline__ int slot(int t, int q) const { return t * wstride + q; }   // q = (p << rsh) + copy
     __device__ __forceinline__ unsigned long long sum(int t, int p) const {
         if (rsh == 0) return acc[slot(t, p)];
         unsigned long long v = 0;
@@ -179,14 +165,21 @@ struct AccWindow {
     }
 };
 
-// Copies per entry of a window of W entries in wstride slots (a power of
-// two, at most 64).
+// Copies per entry of a window of W entries in wstride slots: a power of
+// two, at most 64.
 __device__ __forceinline__ int acc_copies_log2(int W, int wstride) {
-    if (!MC_ACC_REPL || W <= 0) return 0;
+    if (W <= 0) return 0;
     const int q = wstride / W;
     return q <= 1 ? 0 : min(6, 31 - __builtin_clz((unsigned)q));
 }
 
+// --------------------------------------------------------------------------
+// Emitters: where a per-qd expected count goes.
+// --------------------------------------------------------------------------
+// add(t, r, p, x): expected count x, a posterior in [0, 1], of per-qd table
+// t for the entry p shown at rank r of the current session; add_wide takes
+// any non-negative x (the CCM's last-click attractiveness term can exceed 1;
+// the last-click terms, once per session, always take it).
 template <int T>
 struct EmitLds {            // fixed-point LDS window, relative indices
     AccWindow<T> w;
