@@ -212,6 +212,22 @@ mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uin
 mc_status mc_evaluate_test_set(mc_model* m, float* logp, float* full, float* llh, double* llh_sum,
                                double* ppl_log2_sum, int64_t* rank_count, double* kernel_ms);
 
+/* mc_evaluate for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  Every array is a DEVICE pointer: pidx[n*10],
+ * clicks[n*10] (session-major as in mc_evaluate), len[n] (NULL = 10),
+ * logp/full (n*10) and llh[n] (each nullable), totals[21] = llh_sum,
+ * ppl_log2_sum[10], rank_count[10] (as doubles).  Pair indices outside
+ * [-1, n_qd) read as unseen pairs and lengths are clamped to [1, 10], since
+ * the host cannot check device arrays.  The call enqueues the evaluation on
+ * the model's stream and returns without waiting (mc_synchronize waits); it
+ * calls no hipMalloc, hipFree or allocating routine.  Its FP64 block
+ * partials live in a fixed device array of the library, 344 KB on each GPU
+ * it runs on, held until the process ends: evaluations that overlap on one
+ * GPU (models with different streams) race on it.  n at most 2^31 - 1. */
+mc_status mc_evaluate_device(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                             float* logp, float* full, float* llh, double* totals);
+
 /* get_memory_usage (base.cuh:31): device bytes held by the model. */
 mc_status mc_get_memory_usage(const mc_model* m, size_t* bytes);
 

@@ -43,6 +43,7 @@ _SIGS = {
     "mc_serp_probability": (C.c_int, [vp, C.c_int64, i32p, u8p, u8p, f32p]),
     "mc_set_test_set": (C.c_int, [vp, C.c_int64, i32p, u8p, u8p]),
     "mc_evaluate_test_set": (C.c_int, [vp, f32p, f32p, f32p, f64p, f64p, i64p, f64p]),
+    "mc_evaluate_device": (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
     "mc_get_memory_usage": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
     "mc_compute_memory_footprint": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_size_t)]),
     "mc_get_layout_info": (C.c_int, [vp, i64p, i64p, i64p]),

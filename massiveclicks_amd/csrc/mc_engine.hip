@@ -827,6 +827,11 @@ mc_status mc_model_create(int model_type, int device, mc_model** out) {
     m->type = model_type;
     m->device = device;
     shape_of(model_type, &m->T, &m->K, &m->KACC);
+    const mc_status ss = eval_scratch_setup(device);
+    if (ss != MC_OK) {
+        delete m;
+        return ss;
+    }
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete m;

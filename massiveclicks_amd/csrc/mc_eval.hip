@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -306,6 +307,40 @@ constexpr int64_t kEvalMaxBlocks = 2048;
 
 int64_t eval_blocks(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, kEvalMaxBlocks)); }
 
+// Scratch of mc_evaluate_device, the entry point for callers that cannot
+// allocate device memory: the per-block FP64 partials of one evaluation
+// (kEvalMaxBlocks x 21 doubles, 344 KB per device, held until the process
+// ends).  Every block writes all 21 of its partials before the reduction
+// reads them, so nothing has to be cleared between calls.  Host code never
+// passes `scratch` itself to a kernel: its device address on each GPU comes
+// from hipGetSymbolAddress (eval_scratch_setup).
+constexpr size_t kScratchDoubles = (size_t)kEvalMaxBlocks * 21;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Largest n mc_evaluate_device takes (the kernel strides over any n with at
+// most kEvalMaxBlocks blocks; the bound keeps session indices in 31 bits).
+constexpr int64_t kEvalDeviceMaxSessions = INT32_MAX;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+mc_status eval_scratch_setup(int device) {
+    MC_REQUIRE(device >= 0 && device < kMaxDevices, "device ordinal out of range of the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MC_OK;
+    MC_HIP(hipSetDevice(device));
+    void* ptr = nullptr;
+    MC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (double*)ptr;
+    return MC_OK;
+}
+
+double* eval_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+
 template <bool RANK_MAJOR, bool OUT>
 void launch_eval_out(int type, const EvalArgs& E, int64_t nb, hipStream_t s) {
     switch (type) {
@@ -442,6 +477,33 @@ mc_status mc_serp_probability(mc_model* m, int64_t n, const int32_t* pidx, const
     MC_TRY(check_sessions(m, n, pidx, clicks, len));
     double out[21];
     return evaluate_host_arrays(m, n, pidx, clicks, len, nullptr, nullptr, nullptr, prob, out);
+}
+
+mc_status mc_evaluate_device(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
+                             float* logp, float* full, float* llh, double* totals) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    MC_REQUIRE(pidx && clicks && totals, "null argument");
+    MC_REQUIRE(n >= 1 && n <= kEvalDeviceMaxSessions,
+               "n out of range [1, " + std::to_string(kEvalDeviceMaxSessions) + "] for mc_evaluate_device");
+    const int64_t nb = eval_blocks(n);
+    MC_REQUIRE((size_t)nb * 21 <= kScratchDoubles, "internal: evaluation partials exceed the scratch array");
+    double* part = eval_scratch(m->device);
+    MC_REQUIRE(part, "internal: evaluation scratch not set up on this device");
+    MC_HIP(hipSetDevice(m->device));
+    EvalArgs E = model_eval_args(m);
+    E.pidx = pidx;
+    E.clicks = clicks;
+    E.len = len;
+    E.n = n;
+    E.logp = logp;
+    E.full = full;
+    E.llh = llh;
+    E.partial = part;
+    launch_eval<false>(m->type, E, nb, m->stream);
+    MC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, (const double*)part, (int)nb, totals);
+    MC_HIP(hipGetLastError());
+    return MC_OK;
 }
 
 mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len) {

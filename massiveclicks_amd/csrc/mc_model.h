@@ -152,6 +152,10 @@ void release(mc_model* m);
 // Fills n floats with v on m's stream (a kernel, safe inside graph capture).
 mc_status fill(mc_model* m, float* p, float v, int64_t n);
 const char* table_name(int type, int t);
+// Resolves, once per device, the device address of the evaluation scratch
+// array of mc_evaluate_device (mc_eval.hip); called by mc_model_create with
+// the device current.
+mc_status eval_scratch_setup(int device);
 
 }  // namespace mc
 
