@@ -172,6 +172,12 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
                 break
     log(f"time-to-target: EM 50 it {em_ms:.2f} ms -> LL {target:.6f}, first in band at iteration {em_first} "
         f"({em_first_ms:.3f} ms); Adam {curve[-1]}")
+    # Algorithmic bytes of an Adam epoch (every session and entry once): per
+    # session the 32-bit index of each shown rank and the 2-byte mask; per
+    # entry and per-qd table the logit and both moments read, and the logit,
+    # both moments and the value written (28 B).
+    epoch_bytes = ds.n_train * (4 * 10 + 2) + 28 * 2 * ds.n_qd
+    adam_frac = (epoch_bytes * reached[0] / (reached[1] * 1e-3) / HBM_PEAK) if reached else None
     return {
         "workload": f"DBN, {n_sessions} synthetic sessions ({ds.n_train} train), 1M Zipf(1.1) queries; Adam lr {lr}, "
                     f"{batches} minibatches of whole query groups (~{batch_sessions} sessions), log-prior on, "
@@ -188,6 +194,8 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
         "adam_ll": reached[2] if reached else curve[-1][2],
         "speedup_vs_em": (em_ms / reached[1]) if reached else None,
         "speedup_vs_em_first": (em_first_ms / reached[1]) if reached else None,
+        "adam_epoch_bytes": epoch_bytes,
+        "adam_frac": adam_frac,
         "adam_curve": curve,
     }
 
