@@ -18,7 +18,7 @@ CLI = os.path.join(LIBDIR, "mclicks")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["mc_engine.hip", "mc_eval.hip", "mc_grad.hip", "mc_dataset.cpp", "mc_io.cpp", "mc_tune.cpp"]
+SOURCES = ["mc_engine.hip", "mc_eval.hip", "mc_grad.hip", "mc_layout.cpp", "mc_dataset.cpp", "mc_io.cpp", "mc_tune.cpp"]
 
 # -ffp-contract=off keeps the E-step arithmetic in the reference's operation
 # order; packed FP32 instructions are disabled on the device.

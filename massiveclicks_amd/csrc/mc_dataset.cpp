@@ -503,3 +503,110 @@ bool parse_raw_log(const char* path, int max_sessions, std::vector<int32_t>* q_a
 }
 
 }  // namespace mc
+
+// ===========================================================================
+// C ABI of the dataset (include/mclicks.h)
+// ===========================================================================
+using namespace mc;
+
+extern "C" {
+
+mc_status mc_dataset_parse(const char* path, int max_sessions, float test_share, mc_dataset** out) {
+    MC_REQUIRE(path && out, "null argument");
+    MC_REQUIRE(max_sessions == -1 || max_sessions > 0, "max_sessions must be -1 or positive");
+    std::vector<int32_t> q, d;
+    std::vector<uint8_t> c;
+    std::string err;
+    try {
+        if (!parse_raw_log(path, max_sessions, &q, &d, &c, &err)) {
+            set_error(err);
+            return err.rfind("cannot open", 0) == 0 ? MC_ERR_IO : MC_ERR_INVALID_ARG;
+        }
+        mc_dataset* ds = new mc_dataset;
+        if (!prepare_dataset(ds->d, q, d, c, test_share, &err)) {
+            delete ds;
+            set_error(err);
+            return MC_ERR_INVALID_ARG;
+        }
+        *out = ds;
+    } catch (std::bad_alloc&) {
+        set_error("out of host memory while preparing the dataset");
+        return MC_ERR_OOM;
+    }
+    return MC_OK;
+}
+
+mc_status mc_dataset_from_sessions(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
+                                   float test_share, mc_dataset** out) {
+    return mc_dataset_from_sessions_len(n, query, docs, clicks, nullptr, test_share, out);
+}
+
+mc_status mc_dataset_from_sessions_len(int64_t n, const int32_t* query, const int32_t* docs, const uint8_t* clicks,
+                                       const uint8_t* len, float test_share, mc_dataset** out) {
+    MC_REQUIRE(out && n > 0 && query && docs && clicks, "invalid argument");
+    MC_REQUIRE(n <= INT32_MAX, "too many sessions");
+    if (len) {
+        for (int64_t s = 0; s < n; s++) {
+            MC_REQUIRE(len[s] >= 1 && len[s] <= kMaxSerp, "len must be in [1, 10] (session " + std::to_string(s) + ")");
+            for (int r = len[s]; r < kMaxSerp; r++)
+                MC_REQUIRE(clicks[s * kMaxSerp + r] == 0, "click on a rank past len (session " + std::to_string(s) + ")");
+        }
+    }
+    std::string err;
+    try {
+        std::vector<int32_t> q(query, query + n), d(docs, docs + n * kMaxSerp);
+        std::vector<uint8_t> c(clicks, clicks + n * kMaxSerp);
+        std::vector<uint8_t> l;
+        if (len) l.assign(len, len + n);
+        mc_dataset* ds = new mc_dataset;
+        if (!prepare_dataset(ds->d, q, d, c, test_share, &err, len ? &l : nullptr)) {
+            delete ds;
+            set_error(err);
+            return MC_ERR_INVALID_ARG;
+        }
+        *out = ds;
+    } catch (std::bad_alloc&) {
+        set_error("out of host memory while preparing the dataset");
+        return MC_ERR_OOM;
+    }
+    return MC_OK;
+}
+
+mc_status mc_dataset_sizes(const mc_dataset* ds, int64_t* n_parsed, int64_t* n_train, int64_t* n_test,
+                           int64_t* n_qd) {
+    MC_REQUIRE(ds, "null dataset");
+    if (n_parsed) *n_parsed = ds->d.n_parsed;
+    if (n_train) *n_train = (int64_t)ds->d.train_query.size();
+    if (n_test) *n_test = (int64_t)ds->d.test_query.size();
+    if (n_qd) *n_qd = ds->d.n_qd;
+    return MC_OK;
+}
+
+mc_status mc_dataset_export(const mc_dataset* ds, int which, int32_t* query, int32_t* docs, uint8_t* clicks,
+                            int32_t* pidx) {
+    MC_REQUIRE(ds && (which == 0 || which == 1), "invalid argument");
+    const HostDataset& d = ds->d;
+    const auto& Q = which == 0 ? d.train_query : d.test_query;
+    const auto& D = which == 0 ? d.train_docs : d.test_docs;
+    const auto& C = which == 0 ? d.train_clicks : d.test_clicks;
+    const auto& P = which == 0 ? d.train_pidx : d.test_pidx;
+    if (query) std::copy(Q.begin(), Q.end(), query);
+    if (docs) std::copy(D.begin(), D.end(), docs);
+    if (clicks) std::copy(C.begin(), C.end(), clicks);
+    if (pidx) std::copy(P.begin(), P.end(), pidx);
+    return MC_OK;
+}
+
+mc_status mc_dataset_export_len(const mc_dataset* ds, int which, uint8_t* len) {
+    MC_REQUIRE(ds && (which == 0 || which == 1) && len, "invalid argument");
+    const HostDataset& d = ds->d;
+    const auto& L = which == 0 ? d.train_len : d.test_len;
+    const size_t n = which == 0 ? d.train_query.size() : d.test_query.size();
+    if (L.empty()) std::fill(len, len + n, (uint8_t)kMaxSerp);
+    else std::copy(L.begin(), L.end(), len);
+    return MC_OK;
+}
+
+void mc_dataset_free(mc_dataset* ds) { delete ds; }
+
+}  // extern "C"

@@ -6,6 +6,8 @@
 #include <string>
 #include <vector>
 
+#include "mclicks.h"
+
 namespace mc {
 
 constexpr int kMaxSerp = 10;   // utils/definitions.h:50
@@ -32,6 +34,21 @@ bool prepare_dataset(HostDataset& ds, const std::vector<int32_t>& q_all, const s
 void set_error(const std::string& msg);
 
 }  // namespace mc
+
+// Argument checks of the entry points: record the message, return a status.
+#define MC_REQUIRE(cond, msg)                                                                 \
+    do {                                                                                      \
+        if (!(cond)) {                                                                        \
+            set_error(msg);                                                                   \
+            return MC_ERR_INVALID_ARG;                                                        \
+        }                                                                                     \
+    } while (0)
+
+#define MC_TRY(expr)                        \
+    do {                                    \
+        mc_status s_ = (expr);              \
+        if (s_ != MC_OK) return s_;         \
+    } while (0)
 
 struct mc_dataset {
     mc::HostDataset d;

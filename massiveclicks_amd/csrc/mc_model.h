@@ -21,19 +21,6 @@
         }                                                                                     \
     } while (0)
 
-#define MC_REQUIRE(cond, msg)                                                                 \
-    do {                                                                                      \
-        if (!(cond)) {                                                                        \
-            set_error(msg);                                                                   \
-            return MC_ERR_INVALID_ARG;                                                        \
-        }                                                                                     \
-    } while (0)
-
-#define MC_TRY(expr)                        \
-    do {                                    \
-        mc_status s_ = (expr);              \
-        if (s_ != MC_OK) return s_;         \
-    } while (0)
 
 struct mc_model {
     int type = 0;
