@@ -202,6 +202,9 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     const AccWindow<T> aw{lacc, Wmax, PATH == kPathLds ? acc_copies_log2(W, Wmax) : 0};
     const int WS = W << aw.rsh;
     float* lval = fwin ? facc + T * W : (float*)(lds + T * Wmax);
+    // UBM examination slots in LDS after the window (fixed-point window path).
+    constexpr bool lslots = MODEL == kUBM && PATH == kPathLds;
+    unsigned long long* slots = lds + ((size_t)T * Wmax * 12 + 7) / 8;
 
     // First session's loads go out before the window is staged.
     const Stream st = make_stream(A);
@@ -223,6 +226,8 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 #pragma unroll
                 for (int t = 0; t < T; t++) lacc[aw.slot(t, i)] = 0ull;
     }
+    if (lslots)
+        for (int i = tid; i < kUbmSlots * kUbmCopies; i += nthr) slots[i] = 0ull;
     __syncthreads();
 
     float acc[55];
@@ -234,7 +239,7 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
         EmitGlobalFloat em{A.tab[0].gaccf, A.tab[1].gaccf};
         run_item<MODEL, VARLEN, false>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLds) {
-        EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1)};
+        EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1), slots + (tid & (kUbmCopies - 1))};
         const SrcLds<T> lsrc{lval};
         run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLdsFloat) {
@@ -260,6 +265,8 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     for (int k = tid; k < KACC; k += nthr) {
         long long v = 0;
         for (int w = 0; w < nwave; w++) v += red[k][w];
+        if (lslots)
+            for (int c = 0; c < kUbmCopies; c++) v += (long long)slots[k * kUbmCopies + c];
         if (v != 0) atomicAdd(&A.sh_acc[k], (unsigned long long)v);
     }
 
@@ -344,6 +351,11 @@ em_window_persistent_kernel(KArgs A, int n_win) {
     const int Wmax = A.max_width;
     unsigned long long* lacc = lds;
     float* lval = (float*)(lds + T * Wmax);
+    // UBM examination slots in LDS after the window, kept across items.
+    constexpr bool lslots = MODEL == kUBM;
+    unsigned long long* slots = lds + ((size_t)T * Wmax * 12 + 7) / 8;
+    if (lslots)
+        for (int i = tid; i < kUbmSlots * kUbmCopies; i += NT) slots[i] = 0ull;
     const Stream st = make_stream(A);
     for (int k = tid; k < K; k += NT) sh[k] = A.sh_val_in[k];
 
@@ -408,7 +420,7 @@ em_window_persistent_kernel(KArgs A, int n_win) {
             cnt[u] = (i < W) ? A.counts16[it.base + i] : 0u;
         }
         {
-            EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1)};
+            EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1), slots + (tid & (kUbmCopies - 1))};
             const SrcLds<T> lsrc{lval};
             run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
         }
@@ -458,7 +470,9 @@ em_window_persistent_kernel(KArgs A, int n_win) {
     }
     __syncthreads();
     for (int k = tid; k < KACC; k += NT) {
-        const long long v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        long long v = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+        if (lslots)
+            for (int c = 0; c < kUbmCopies; c++) v += (long long)slots[k * kUbmCopies + c];
         if (v != 0) atomicAdd(&A.sh_acc[k], (unsigned long long)v);
     }
 }

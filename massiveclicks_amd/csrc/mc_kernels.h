@@ -180,18 +180,30 @@ __device__ __forceinline__ int acc_copies_log2(int W, int wstride) {
 // t for the entry p shown at rank r of the current session; add_wide takes
 // any non-negative x (the CCM's last-click attractiveness term can exceed 1;
 // the last-click terms, once per session, always take it).
+// UBM examination counts of a workgroup in LDS: kUbmSlots fixed-point
+// accumulators, each in kUbmCopies copies; lane l adds into copy
+// l mod kUbmCopies, so the lanes of one instruction never share an address
+// with more than one other lane.  Laid out [slot][copy].
+constexpr int kUbmSlots = 55;
+constexpr int kUbmCopies = 32;
+constexpr size_t kUbmSlotBytes = (size_t)kUbmSlots * kUbmCopies * 8;
+
 template <int T>
 struct EmitLds {            // fixed-point LDS window, relative indices
     AccWindow<T> w;
     int copy;                  // this lane's copy
+    unsigned long long* slots; // UBM: this lane's copy of the examination slots (stride kUbmCopies)
+    static constexpr bool kSlotsInLds = true;
     __device__ __forceinline__ void add(int t, int, int p, float x) {
         atomicAdd(&w.acc[w.slot(t, (p << w.rsh) + copy)], to_fix_unit(x));
     }
     __device__ __forceinline__ void add_wide(int t, int, int p, float x) {
         atomicAdd(&w.acc[w.slot(t, (p << w.rsh) + copy)], (unsigned long long)to_fix(x));
     }
+    __device__ __forceinline__ void add_slot(int k, float x) { atomicAdd(&slots[k * kUbmCopies], to_fix_unit(x)); }
 };
 struct EmitGlobalFix {      // fixed-point global (item without a window)
+    static constexpr bool kSlotsInLds = false;
     unsigned long long* g0;
     unsigned long long* g1;
     __device__ __forceinline__ void add(int t, int, int p, float x) {
@@ -203,11 +215,13 @@ struct EmitGlobalFix {      // fixed-point global (item without a window)
 };
 template <int T>
 struct EmitLdsFloat {       // float LDS window (float-atomic mode), relative indices
+    static constexpr bool kSlotsInLds = false;
     float* acc;                // [width][T]
     __device__ __forceinline__ void add(int t, int, int p, float x) { atomicAdd(&acc[p * T + t], x); }
     __device__ __forceinline__ void add_wide(int t, int r, int p, float x) { add(t, r, p, x); }
 };
 struct EmitGlobalFloat {    // float atomics (reference-like mode)
+    static constexpr bool kSlotsInLds = false;
     float* g0;
     float* g1;
     __device__ __forceinline__ void add(int t, int, int p, float x) {
@@ -219,6 +233,7 @@ struct EmitGlobalFloat {    // float atomics (reference-like mode)
 // session): the counts stay in registers, indexed by rank, and the lane
 // writes the new parameter values itself.
 struct EmitRegs {
+    static constexpr bool kSlotsInLds = false;
     float x[2][kL];
     __device__ __forceinline__ void clear() {
 #pragma unroll
@@ -324,6 +339,7 @@ __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t
     }
     int slot[kL];
     ubm_slots(mm, slot);
+    const int fc = __builtin_ctz(mm);   // first click; ranks before it have no earlier click
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         if (r < L && !((mm >> r) & 1u)) {
@@ -336,9 +352,17 @@ __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t
             float na = (atr - ae) * inv;
             float ne = (ex - ae) * inv;
             em.add(0, r, p, na);
+            if constexpr (Emit::kSlotsInLds) {
+                // Before the first click the slot is the row's "no earlier
+                // click" one, known at compile time (a register, as for
+                // click-free sessions); after it, one LDS add into the slot.
+                if (r < fc) acc[row + kL - 1] += ne;
+                else em.add_slot(slot[r], ne);
+            } else {
 #pragma unroll
-            for (int j = 0; j < r; j++) acc[row + j] += (slot[r] == row + j) ? ne : 0.f;
-            acc[row + kL - 1] += (slot[r] == row + kL - 1) ? ne : 0.f;
+                for (int j = 0; j < r; j++) acc[row + j] += (slot[r] == row + j) ? ne : 0.f;
+                acc[row + kL - 1] += (slot[r] == row + kL - 1) ? ne : 0.f;
+            }
         }
     }
 }

@@ -345,6 +345,9 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
     for (const Item& it : items) max_width = std::max(max_width, (int)it.width);
     m->max_width = max_width;
     m->lds_bytes = (size_t)max_width * T * (m->reduce_mode == MC_REDUCE_SEGMENTED ? 12 : 8);
+    // The UBM's fixed-point windows are followed by its examination slots.
+    if (m->type == MC_UBM && m->reduce_mode == MC_REDUCE_SEGMENTED)
+        m->lds_bytes = (m->lds_bytes + 7) / 8 * 8 + kUbmSlotBytes;
     // ---- device layout: rank-major pidx, click masks, lengths ---------------
     // Window items come first in device order.  Their sessions store 16-bit
     // indices relative to the window base (the kernels address the LDS
