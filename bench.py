@@ -14,6 +14,8 @@ HBM.  Prints one JSON line:
               training set, on 16 host threads (one GPU's share of the
               host), without and with the reference's per-thread copy of
               the training set
+  other_models  CCM, UBM and PBM on their own 10M-session workloads: time
+              per iteration and the window kernel's roofline fraction
   time_to_target  DBN on 20M sessions: wall time of plain EM's 50
               iterations (and of its first iteration in the target band),
               and of Adam on the log-likelihood gradient (lr 0.05,
@@ -200,6 +202,38 @@ def time_to_target(n_sessions, batch_sessions, lr, max_epochs, seed=5):
     }
 
 
+def other_models(models, n_sessions, steps, warmup):
+    """The other click models on their 10M-session workloads (same
+    generator, each model's own ground truth, segmented reduction): time per
+    graph-replayed iteration, the window kernel's event-timed duration and
+    its fraction of the HBM roofline (algorithmic bytes as for the DBN, one
+    per-qd table)."""
+    from massiveclicks_amd import Dataset, ClickModel, synth
+    from massiveclicks_amd.model import PBM, UBM, CCM
+    ids = {"pbm": PBM, "ubm": UBM, "ccm": CCM}
+    out = {}
+    for name in models:
+        t0 = time.perf_counter()
+        s = synth.generate(synth.SynthConfig(n_sessions=n_sessions, n_queries=1_000_000, vocab=20_000_000,
+                                             zipf_s=1.1, model=name, seed=3))
+        ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2)
+        del s
+        with ClickModel(ids[name]) as cm:
+            cm.init_from_dataset(ds)
+            cm.em(warmup, use_graph=True)
+            ms_iter = cm.em_graph_timed(steps) / steps
+            kt = cm.em_timed(min(steps, 10))
+            n_shared = cm.layout_info()["n_shared_entries"]
+        item_bytes = ds.n_train * 42 + 16 * (ds.n_qd - n_shared)
+        out[name] = {"ms_per_step": ms_iter, "value": ds.n_train / (ms_iter * 1e-3), "kernel_ms": kt["ms_estep"],
+                     "bytes_per_launch": item_bytes, "frac": item_bytes / (kt["ms_estep"] * 1e-3) / HBM_PEAK,
+                     "train_sessions": ds.n_train, "n_qd": ds.n_qd}
+        log(f"{name.upper()} 10M: {ms_iter:.4f} ms/iter, kernel {kt['ms_estep']:.4f} ms, "
+            f"frac {out[name]['frac']:.3f} ({time.perf_counter() - t0:.1f} s)")
+        del ds
+    return out
+
+
 def calibration_copy(mib, reps=1):
     """Device-to-device copies of `mib` MiB: a plain copy kernel of known size
     (for scaling the FETCH_SIZE / WRITE_SIZE counters) and the copy bandwidth
@@ -256,6 +290,8 @@ def main():
     ap.add_argument("--item-sessions", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--calib-mib", type=int, default=0, help="also run a D2D copy of this size")
+    ap.add_argument("--other-models", default="ccm,ubm,pbm",
+                    help="also time these models on their 10M workloads ('' = none)")
     ap.add_argument("--ttt-sessions", type=int, default=20_000_000, help="time-to-target workload (0 = skip)")
     ap.add_argument("--ttt-batch", type=int, default=1_000_000)
     ap.add_argument("--ttt-lr", type=float, default=0.05)
@@ -420,6 +456,8 @@ def main():
     }
     cm.close()
     del ds
+    if args.other_models and args.model == "dbn" and not batched:
+        out["other_models"] = other_models(args.other_models.split(","), args.sessions, args.steps, args.warmup)
     if args.ttt_sessions and args.model == "dbn":
         out["time_to_target"] = time_to_target(args.ttt_sessions, args.ttt_batch, args.ttt_lr, args.ttt_max_epochs)
     if args.calib_mib:
