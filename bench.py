@@ -371,14 +371,18 @@ def main():
     item_bytes = sess_bytes + 16 * n_tab * (ds.n_qd - n_shared) if segmented else sess_bytes
     iter_bytes = sess_bytes + 16 * n_tab * ds.n_qd
     achieved = item_bytes / (kt["ms_estep"] * 1e-3)
-    # What this engine's own layout moves per launch: 2 B per shown rank for
-    # sessions of window items (16-bit window-relative indices), 4 B for the
-    # others, the mask (and length), and per entry an item owns the value in
-    # and out of each table plus its 2-byte packed counts.
+    # What this engine's own layout moves per launch: 16 B per session of a
+    # packed window item (offsets, base, clicks and length in one record),
+    # 2 B per shown rank for sessions of the other window items (16-bit
+    # window-relative indices), 4 B for the rest, with the mask (and length),
+    # and per entry an item owns the value in and out of each table plus its
+    # 2-byte packed counts.
     si = cm.stream_info()
     n_w = si["n_window_sessions"]
+    n_pk = si["n_packed"]
     extra = 2 + (1 if batched else 0)
-    layout_sess = n_w * (2 * mean_len + extra) + (n_train - n_w) * (4 * mean_len + extra)
+    layout_sess = (n_pk * 16 + (n_w - n_pk) * (2 * mean_len + extra) +
+                   (n_train - n_w) * (4 * mean_len + extra))
     layout_bytes = layout_sess + (8 * n_tab + 2) * (ds.n_qd - n_shared) if segmented else layout_sess
 
     # Held-out evaluation on the resident test set: the evaluation kernel's
@@ -443,7 +447,7 @@ def main():
                      "bytes_per_launch": item_bytes,
                      "layout_bytes_per_launch": layout_bytes,
                      "layout_frac": layout_bytes / (kt["ms_estep"] * 1e-3) / HBM_PEAK,
-                     "window_sessions": n_w,
+                     "window_sessions": n_w, "packed_sessions": n_pk,
                      "kernel_ms": kt["ms_estep"], "merge_kernel_ms": kt["ms_mstep"],
                      "iteration_bytes": iter_bytes,
                      "iteration_frac": iter_bytes / (ms_iter * 1e-3) / HBM_PEAK},

@@ -63,7 +63,13 @@ enum {
     /* Minibatches of the gradient path (mc_adam_steps): the query-sorted
      * training set is cut into this many runs of whole query groups of about
      * equal session counts (default 1 = full batch). */
-    MC_OPT_BATCHES = 7
+    MC_OPT_BATCHES = 7,
+    /* 1 = in the segmented mode, window items whose sessions each show
+     * documents within 256 consecutive entries of the window store every
+     * session as one 16-byte record (8-bit offsets from the session's
+     * smallest index, the click bits and the length), read with one load;
+     * 0 = every window item keeps 16-bit rank-major indices (default 1). */
+    MC_OPT_PACKED = 8
 };
 enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };
 
@@ -240,13 +246,19 @@ mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_sha
                              int64_t* max_item_sessions);
 
 /* Session stream chosen at init (no counterpart in the reference, which
- * keeps a 32-bit index per rank): sessions of window items store 16-bit
- * window-relative indices, the others 32-bit absolute ones.  Returns the
- * number of window-item sessions and the device bytes of all indices. */
+ * keeps a 32-bit index per rank): sessions of window items store
+ * window-relative indices (16-byte packed records or 16-bit indices), the
+ * others 32-bit absolute ones.  Returns the number of window-item sessions
+ * and the device bytes of all indices (packed records counted whole). */
 mc_status mc_get_stream_info(const mc_model* m, int64_t* n_window_sessions, int64_t* index_bytes);
 
+/* Sessions per stored format (no counterpart in the reference): packed
+ * 16-byte records, 16-bit window-relative rank-major indices, 32-bit
+ * absolute rank-major indices (MC_OPT_PACKED). */
+mc_status mc_get_session_formats(const mc_model* m, int64_t* n_packed, int64_t* n_window16, int64_t* n_absolute);
+
 /* Number of mc_model_set_option options (indices 0 .. MC_N_OPTIONS - 1). */
-#define MC_N_OPTIONS 8
+#define MC_N_OPTIONS 9
 
 /* Autotuner of the EM work decomposition; no counterpart in the reference
  * (whose launch shape is fixed: BLOCK_SIZE 128, one session per thread).
@@ -255,8 +267,8 @@ mc_status mc_get_stream_info(const mc_model* m, int64_t* n_window_sessions, int6
  * measured as `iterations` EM iterations replayed from a graph on `device`
  * on this training set (arguments as mc_init_parameters): the reduction mode
  * first, then sessions per work item (2048..16384, i.e. sessions per lane),
- * the LDS window and the launch shape (persistent kernel, or one workgroup
- * of 256 or 512 lanes per item).  *best_ms_per_iter and *n_trials report the
+ * the LDS window, the launch shape (persistent kernel, or one workgroup
+ * of 256 or 512 lanes per item) and the session format (MC_OPT_PACKED).  *best_ms_per_iter and *n_trials report the
  * winner's time and the number of candidates measured. */
 mc_status mc_autotune(int model_type, int device, int64_t n, const int32_t* query, const int32_t* pidx,
                       const uint8_t* clicks, const uint8_t* len, int64_t n_qd, int iterations, int64_t* options,

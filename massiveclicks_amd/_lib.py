@@ -48,6 +48,7 @@ _SIGS = {
     "mc_compute_memory_footprint": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_size_t)]),
     "mc_get_layout_info": (C.c_int, [vp, i64p, i64p, i64p]),
     "mc_get_stream_info": (C.c_int, [vp, i64p, i64p]),
+    "mc_get_session_formats": (C.c_int, [vp, i64p, i64p, i64p]),
     "mc_get_model_type": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "mc_autotune": (C.c_int, [C.c_int, C.c_int, C.c_int64, i32p, i32p, u8p, u8p, C.c_int64, C.c_int, i64p, f64p,
                               C.POINTER(C.c_int)]),

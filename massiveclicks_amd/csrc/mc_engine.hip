@@ -131,6 +131,24 @@ __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const
     }
 }
 
+// run_item for packed items: one 16-byte record per session; the next
+// session's record is in flight while one is processed (4 registers
+// instead of the 12 of a rank-major session).
+template <int MODEL, bool VARLEN, class Src, class Emit>
+__device__ __forceinline__ void run_item_packed(const Stream& st, const Item& it, const Src& src, Emit& em,
+                                                const float* sh, float (&acc)[55], PackedSession cur) {
+    const int nthr = blockDim.x;
+    for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
+        const PackedSession nxt = load_packed(st, s + nthr, it.s1);
+        uint32_t m;
+        int L;
+        int pv[kL];
+        decode_packed<VARLEN>(cur, m, L, pv);
+        run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
+        cur = nxt;
+    }
+}
+
 // Direct items: every session is the only one of its query, so its entries
 // receive no other counts; the lane writes theta_{t+1} of its entries itself.
 template <int MODEL, bool VARLEN, class Src>
@@ -169,9 +187,10 @@ __device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st
 }
 
 // Session paths, one kernel instantiation each (each gets its own register
-// allocation): window items in LDS, items without a window (global 64-bit
-// fixed-point atomics), and the float-atomic mode.
-enum { kPathLds = 0, kPathGlobal = 1, kPathAtomic = 2, kPathDirect = 3, kPathLdsFloat = 4 };
+// allocation): window items in LDS (rank-major or packed sessions), items
+// without a window (global 64-bit fixed-point atomics), and the float-atomic
+// mode.
+enum { kPathLds = 0, kPathGlobal = 1, kPathAtomic = 2, kPathDirect = 3, kPathLdsFloat = 4, kPathPacked = 5 };
 
 // E-step and in-item M-step of one work item (item index item_offset +
 // blockIdx.x).  Dynamic LDS (window path): width x T fixed-point
@@ -193,25 +212,29 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     const int nthr = blockDim.x;
     const int W = it.width;
     constexpr bool fwin = PATH == kPathLdsFloat;
-    constexpr bool win = PATH == kPathLds || fwin;
+    constexpr bool packed = PATH == kPathPacked;
+    constexpr bool fixwin = PATH == kPathLds || packed;   // fixed-point LDS window
+    constexpr bool win = fixwin || fwin;
     unsigned long long* lacc = lds;
     float* facc = (float*)lds;
     // Fixed-point windows take max_width slots per table (copies of the
     // entries of narrow windows), then the staged values.
     const int Wmax = A.max_width;
-    const AccWindow<T> aw{lacc, Wmax, PATH == kPathLds ? acc_copies_log2(W, Wmax) : 0};
+    const AccWindow<T> aw{lacc, Wmax, fixwin ? acc_copies_log2(W, Wmax) : 0};
     const int WS = W << aw.rsh;
     float* lval = fwin ? facc + T * W : (float*)(lds + T * Wmax);
     // UBM examination slots in LDS after the window (fixed-point window path).
-    constexpr bool lslots = MODEL == kUBM && PATH == kPathLds;
+    constexpr bool lslots = MODEL == kUBM && fixwin;
     unsigned long long* slots = lds + ((size_t)T * Wmax * 12 + 7) / 8;
 
     // First session's loads go out before the window is staged.
     const Stream st = make_stream(A);
-    uint32_t m0;
-    int L0;
+    uint32_t m0 = 0;
+    int L0 = 0;
     int pv0[kL];
-    load_session<VARLEN, win>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    PackedSession r0;
+    if (packed) r0 = load_packed(st, it.s0 + tid, it.s1);
+    else load_session<VARLEN, win>(st, it.s0 + tid, it.s1, m0, L0, pv0);
     for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
     if (win) {
         for (int i = tid; i < W; i += nthr) {
@@ -238,10 +261,11 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     if (PATH == kPathAtomic) {
         EmitGlobalFloat em{A.tab[0].gaccf, A.tab[1].gaccf};
         run_item<MODEL, VARLEN, false>(st, it, gsrc, em, sh, acc, m0, L0, pv0);
-    } else if (PATH == kPathLds) {
+    } else if (fixwin) {
         EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1), slots + (tid & (kUbmCopies - 1))};
         const SrcLds<T> lsrc{lval};
-        run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+        if (packed) run_item_packed<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, r0);
+        else run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
     } else if (PATH == kPathLdsFloat) {
         EmitLdsFloat<T> em{facc};
         const SrcLds<T> lsrc{lval};
@@ -313,7 +337,8 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     }
 }
 
-// Persistent form of the window path: workgroup b of G resident ones
+// Persistent form of the window path over items [first, first + n_win), all
+// packed (PACKED) or all rank-major: workgroup b of G resident ones
 // processes one window item per round, item kG + b in even rounds k and
 // kG + G - 1 - b in odd ones.  Items are ordered largest first, so this
 // back-and-forth order pairs the larger items of one round with the smaller
@@ -336,9 +361,9 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 template <int MODEL, bool VARLEN>
 struct PersistWaves { static constexpr int value = (!VARLEN && (MODEL == kDBN || MODEL == kCCM)) ? 5 : (VARLEN && MODEL == kUBM) ? 3 : 1; };
 
-template <int MODEL, bool VARLEN, int PER>
+template <int MODEL, bool VARLEN, int PER, bool PACKED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWaves<MODEL, VARLEN>::value)))
-em_window_persistent_kernel(KArgs A, int n_win) {
+em_window_persistent_kernel(KArgs A, int first, int n_win) {
     constexpr int T = Shape<MODEL>::T;
     constexpr int K = Shape<MODEL>::K;
     constexpr int KACC = Shape<MODEL>::KACC;
@@ -363,7 +388,7 @@ em_window_persistent_kernel(KArgs A, int n_win) {
     auto item_of_round = [&](int k) { return k * G + ((k & 1) ? G - 1 - (int)blockIdx.x : (int)blockIdx.x); };
     int round = 0;
     int cur = item_of_round(0);
-    Item it = cur < n_win ? A.items[cur] : Item{0, 0, 0, 0, 0, 0};
+    Item it = cur < n_win ? A.items[first + cur] : Item{0, 0, 0, 0, 0, 0};
 
     // Window values of the current item, and its first session.
     float wv[PER][T];
@@ -373,10 +398,12 @@ em_window_persistent_kernel(KArgs A, int n_win) {
 #pragma unroll
         for (int t = 0; t < T; t++) wv[u][t] = (i < it.width) ? A.tab[t].val_in[it.base + i] : 0.f;
     }
-    uint32_t m0;
-    int L0;
+    uint32_t m0 = 0;
+    int L0 = 0;
     int pv0[kL];
-    load_session<VARLEN, true>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    PackedSession r0;
+    if (PACKED) r0 = load_packed(st, it.s0 + tid, it.s1);
+    else load_session<VARLEN, true>(st, it.s0 + tid, it.s1, m0, L0, pv0);
 
     float acc[55];
 #pragma unroll
@@ -401,7 +428,7 @@ em_window_persistent_kernel(KArgs A, int n_win) {
         }
         __syncthreads();
         const int nxt = item_of_round(round + 1);
-        const Item itn = nxt < n_win ? A.items[nxt] : Item{0, 0, 0, 0, 0, 0};
+        const Item itn = nxt < n_win ? A.items[first + nxt] : Item{0, 0, 0, 0, 0, 0};
         // In flight during the session loop: next window values, this
         // window's counts.
         // Ten-rank kernels load the next window after the session loop
@@ -422,7 +449,8 @@ em_window_persistent_kernel(KArgs A, int n_win) {
         {
             EmitLds<T> em{aw, tid & ((1 << aw.rsh) - 1), slots + (tid & (kUbmCopies - 1))};
             const SrcLds<T> lsrc{lval};
-            run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
+            if (PACKED) run_item_packed<MODEL, VARLEN>(st, it, lsrc, em, sh, acc, r0);
+            else run_item<MODEL, VARLEN, true>(st, it, lsrc, em, sh, acc, m0, L0, pv0);
         }
         if (LATE_WINDOW) {
 #pragma unroll
@@ -433,7 +461,8 @@ em_window_persistent_kernel(KArgs A, int n_win) {
             }
         }
         // First session of the next item goes out before the flush.
-        load_session<VARLEN, true>(st, itn.s0 + tid, itn.s1, m0, L0, pv0);
+        if (PACKED) r0 = load_packed(st, itn.s0 + tid, itn.s1);
+        else load_session<VARLEN, true>(st, itn.s0 + tid, itn.s1, m0, L0, pv0);
         __syncthreads();
         const bool interior = (it.flags & kItemInterior) != 0;
 #pragma unroll
@@ -596,12 +625,14 @@ void shape_of(int type, int* T, int* K, int* KACC) {
 KArgs make_args(mc_model* m, int cur, int write_num) {
     KArgs A;
     std::memset(&A, 0, sizeof(A));
+    A.packed = m->d_packed;
     A.pidx16 = m->d_pidx16;
     A.pidx = m->d_pidx;
     A.mask = m->d_mask;
     A.len = m->varlen ? m->d_len : nullptr;
     A.N = m->N;
     A.n_w = m->n_w;
+    A.n_pk = m->n_pk;
     A.items = m->d_items;
     A.n_items = m->n_items;
     A.mode = m->reduce_mode;
@@ -645,6 +676,42 @@ void launch_path(mc_model* m, const KArgs& A, int first, int count, size_t lds) 
 // Window width covered by the persistent kernel's register staging.
 constexpr int kPersistPer = 5;
 
+// Resident workgroups per CU of the model's persistent window kernel.
+template <int MODEL, bool PACKED>
+hipError_t persistent_occupancy_of(const mc_model* m, int* per_cu) {
+    return m->varlen
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                     per_cu, em_window_persistent_kernel<MODEL, true, kPersistPer, PACKED>, 256, m->lds_bytes)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                     per_cu, em_window_persistent_kernel<MODEL, false, kPersistPer, PACKED>, 256, m->lds_bytes);
+}
+
+template <bool PACKED>
+hipError_t persistent_occupancy_pk(const mc_model* m, int* per_cu) {
+    switch (m->type) {
+        case MC_PBM: return persistent_occupancy_of<kPBM, PACKED>(m, per_cu);
+        case MC_UBM: return persistent_occupancy_of<kUBM, PACKED>(m, per_cu);
+        case MC_DBN: return persistent_occupancy_of<kDBN, PACKED>(m, per_cu);
+        default: return persistent_occupancy_of<kCCM, PACKED>(m, per_cu);
+    }
+}
+
+hipError_t persistent_occupancy(const mc_model* m, bool packed, int* per_cu) {
+    return packed ? persistent_occupancy_pk<true>(m, per_cu) : persistent_occupancy_pk<false>(m, per_cu);
+}
+
+template <int MODEL, bool PACKED>
+void launch_persistent(mc_model* m, const KArgs& A, int first, int count) {
+    if (count <= 0) return;
+    const dim3 grid((unsigned)std::min(count, m->persist_grid[PACKED]));
+    if (m->varlen)
+        hipLaunchKernelGGL((em_window_persistent_kernel<MODEL, true, kPersistPer, PACKED>), grid, dim3(256),
+                           m->lds_bytes, m->stream, A, first, count);
+    else
+        hipLaunchKernelGGL((em_window_persistent_kernel<MODEL, false, kPersistPer, PACKED>), grid, dim3(256),
+                           m->lds_bytes, m->stream, A, first, count);
+}
+
 template <int MODEL>
 void launch_item(mc_model* m, const KArgs& A) {
     if (m->reduce_mode == MC_REDUCE_ATOMIC) {
@@ -653,15 +720,13 @@ void launch_item(mc_model* m, const KArgs& A) {
         launch_path<MODEL, kPathLdsFloat>(m, A, 0, m->n_win_items, m->lds_bytes);
         launch_path<MODEL, kPathAtomic>(m, A, m->n_win_items, m->n_glob_items, 0);
     } else {
-        if (m->persistent && m->n_win_items > 0) {
-            if (m->varlen)
-                hipLaunchKernelGGL((em_window_persistent_kernel<MODEL, true, kPersistPer>), dim3(m->persist_grid),
-                                   dim3(256), m->lds_bytes, m->stream, A, m->n_win_items);
-            else
-                hipLaunchKernelGGL((em_window_persistent_kernel<MODEL, false, kPersistPer>), dim3(m->persist_grid),
-                                   dim3(256), m->lds_bytes, m->stream, A, m->n_win_items);
+        const int n_pk = m->n_pk_items, n_rm = m->n_win_items - m->n_pk_items;
+        if (m->persistent) {
+            launch_persistent<MODEL, true>(m, A, 0, n_pk);
+            launch_persistent<MODEL, false>(m, A, n_pk, n_rm);
         } else {
-            launch_path<MODEL, kPathLds>(m, A, 0, m->n_win_items, m->lds_bytes);
+            launch_path<MODEL, kPathPacked>(m, A, 0, n_pk, m->lds_bytes);
+            launch_path<MODEL, kPathLds>(m, A, n_pk, n_rm, m->lds_bytes);
         }
         launch_path<MODEL, kPathGlobal>(m, A, m->n_win_items, m->n_glob_items, 0);
         launch_path<MODEL, kPathDirect>(m, A, m->n_win_items + m->n_glob_items, m->n_dir_items, 0);
@@ -787,6 +852,9 @@ mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
         case MC_OPT_PERSISTENT:
             m->persistent = value != 0;
             break;
+        case MC_OPT_PACKED:
+            m->packed = value != 0;
+            break;
         case MC_OPT_BATCHES:
             MC_REQUIRE(value >= 1 && value <= 4096, "batches out of range [1, 4096]");
             m->n_batches = (int)value;
@@ -850,27 +918,19 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     // registers (width <= 256 * kPersistPer), one workgroup per resident slot.
     m->persistent = m->persistent && m->block == 256 && max_width <= 256 * kPersistPer;
     if (m->persistent && m->n_win_items > 0) {
-        int per_cu = 0, n_cu = 0;
-        hipError_t e1 = hipSuccess;
-        switch (m->type) {
-            case MC_PBM: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kPBM, true, kPersistPer>, 256, m->lds_bytes)
-                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kPBM, false, kPersistPer>, 256, m->lds_bytes); break;
-            case MC_UBM: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kUBM, true, kPersistPer>, 256, m->lds_bytes)
-                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kUBM, false, kPersistPer>, 256, m->lds_bytes); break;
-            case MC_DBN: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kDBN, true, kPersistPer>, 256, m->lds_bytes)
-                                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kDBN, false, kPersistPer>, 256, m->lds_bytes); break;
-            default: e1 = m->varlen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kCCM, true, kPersistPer>, 256, m->lds_bytes)
-                                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, em_window_persistent_kernel<kCCM, false, kPersistPer>, 256, m->lds_bytes); break;
-        }
-        MC_HIP(e1);
+        int n_cu = 0;
         MC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, m->device));
-        // At most 6 workgroups per CU: the ten-rank PBM kernel (71 VGPRs)
-        // fits 7, which measured 4 % slower (0.080 -> 0.084 ms per iteration
-        // over 8M sessions) with one more window staged per CU.
-        per_cu = std::min(per_cu, 6);
-        m->persist_grid = std::max(1, std::min(m->n_win_items, per_cu * n_cu));
+        for (int pk = 0; pk < 2; pk++) {
+            int per_cu = 0;
+            MC_HIP(persistent_occupancy(m, pk != 0, &per_cu));
+            // At most 6 workgroups per CU: the ten-rank PBM kernel (71 VGPRs)
+            // fits 7, which measured 4 % slower (0.080 -> 0.084 ms per
+            // iteration over 8M sessions) with one more window staged per CU.
+            m->persist_grid[pk] = std::max(1, std::min(per_cu, 6) * n_cu);
+        }
     }
 
+    MC_TRY(dalloc(m, &m->d_packed, std::max<size_t>(h.packed.size() * 4, 16)));
     MC_TRY(dalloc(m, &m->d_pidx16, std::max<size_t>(h_pidx16.size() * 2, 4)));
     MC_TRY(dalloc(m, &m->d_pidx, std::max<size_t>(h_pidx.size() * 4, 4)));
     MC_TRY(dalloc(m, &m->d_mask, h_mask.size() * 2));
@@ -896,6 +956,8 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     MC_TRY(dalloc(m, &m->d_sh_acc_map, sh_map.size() * 4));
     MC_TRY(dalloc(m, &m->d_sh_acc, (size_t)KACC * 8));
 
+    if (!h.packed.empty())
+        MC_HIP(hipMemcpyAsync(m->d_packed, h.packed.data(), h.packed.size() * 4, hipMemcpyHostToDevice, m->stream));
     if (!h_pidx16.empty())
         MC_HIP(hipMemcpyAsync(m->d_pidx16, h_pidx16.data(), h_pidx16.size() * 2, hipMemcpyHostToDevice, m->stream));
     if (!h_pidx.empty())
@@ -1189,7 +1251,15 @@ mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_sha
 mc_status mc_get_stream_info(const mc_model* m, int64_t* n_window_sessions, int64_t* index_bytes) {
     MC_REQUIRE(m && m->initialized, "model not initialised");
     if (n_window_sessions) *n_window_sessions = m->n_w;
-    if (index_bytes) *index_bytes = m->n_w * kMaxSerp * 2 + (m->N - m->n_w) * kMaxSerp * 4;
+    if (index_bytes) *index_bytes = m->n_pk * 16 + (m->n_w - m->n_pk) * kMaxSerp * 2 + (m->N - m->n_w) * kMaxSerp * 4;
+    return MC_OK;
+}
+
+mc_status mc_get_session_formats(const mc_model* m, int64_t* n_packed, int64_t* n_window16, int64_t* n_absolute) {
+    MC_REQUIRE(m && m->initialized, "model not initialised");
+    if (n_packed) *n_packed = m->n_pk;
+    if (n_window16) *n_window16 = m->n_w - m->n_pk;
+    if (n_absolute) *n_absolute = m->N - m->n_w;
     return MC_OK;
 }
 

@@ -44,11 +44,12 @@ constexpr int kShParts = 8;
 constexpr int kShAccSlot = kShParts * kShAccStride;
 
 struct GArgs {
+    const uint32_t* packed;
     const uint16_t* pidx16;
     const int32_t* pidx;
     const uint16_t* mask;
     const uint8_t* len;
-    int64_t N, n_w;
+    int64_t N, n_w, n_pk;
     const Item* items;
     const int32_t* lists;
     float* val[2];                 // per-qd values (read; written by Adam)
@@ -356,17 +357,23 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
     float* lval = (float*)(lds + T * G.max_width);
 
     KArgs ka;
+    ka.packed = G.packed;
     ka.pidx16 = G.pidx16;
     ka.pidx = G.pidx;
     ka.mask = G.mask;
     ka.len = G.len;
     ka.N = G.N;
     ka.n_w = G.n_w;
+    ka.n_pk = G.n_pk;
     const Stream st = make_stream(ka);
-    uint32_t m0;
-    int L0;
+    // Window items are packed (16-byte session records) or rank-major.
+    const bool packed = WIN && (it.flags & kItemPacked) != 0;
+    uint32_t m0 = 0;
+    int L0 = 0;
     int pv0[kL];
-    load_session<VARLEN, WIN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    PackedSession r0;
+    if (packed) r0 = load_packed(st, it.s0 + tid, it.s1);
+    else load_session<VARLEN, WIN>(st, it.s0 + tid, it.s1, m0, L0, pv0);
     for (int k = tid; k < K; k += NT) sh[k] = G.sh_val[k];
     if (ADAM && G.prev_batch >= 0) {
         // The previous step's finalisation: every workgroup computes the
@@ -421,7 +428,15 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
         const SrcGlobal gsrc{G.val[0], G.val[1]};
         EmitLdsSigned<T> el{lacc};
         EmitGlobalSigned eg{G.gacc[0], G.gacc[1]};
-        for (int s = it.s0 + tid; s < it.s1; s += NT) {
+        if (packed) {
+            for (int s = it.s0 + tid; s < it.s1; s += NT) {
+                const PackedSession nxt = load_packed(st, s + NT, it.s1);
+                decode_packed<VARLEN>(r0, m0, L0, pv0);
+                grad_session<MODEL, VARLEN, !ADAM>(pv0, L0, m0, lsrc, el, sh, acc, ll);
+                r0 = nxt;
+            }
+        }
+        for (int s = packed ? it.s1 : it.s0 + tid; s < it.s1; s += NT) {
             uint32_t m2;
             int L2;
             int pv2[kL];
@@ -634,12 +649,14 @@ mc_status ensure_grad_buffers(mc_model* m) {
 GArgs make_gargs(mc_model* m, int batch) {
     GArgs G;
     std::memset(&G, 0, sizeof G);
+    G.packed = m->d_packed;
     G.pidx16 = m->d_pidx16;
     G.pidx = m->d_pidx;
     G.mask = m->d_mask;
     G.len = m->varlen ? m->d_len : nullptr;
     G.N = m->N;
     G.n_w = m->n_w;
+    G.n_pk = m->n_pk;
     G.items = m->d_items;
     G.lists = m->d_lists;
     for (int t = 0; t < 2; t++) {

@@ -33,15 +33,18 @@ constexpr double kFixInv = 1.0 / 4294967296.0;
 
 enum { kPBM = 0, kCCM = 1, kDBN = 2, kUBM = 3 };
 
-// Work item flags.
-enum { kItemInterior = 1, kItemDirect = 2 };
+// Work item flags.  kItemPacked: a window item whose sessions are stored as
+// 16-byte records (every session's indices lie within 256 consecutive
+// window entries).
+enum { kItemInterior = 1, kItemDirect = 2, kItemPacked = 4 };
 
 struct Item {
     int32_t s0, s1;     // session range [s0, s1) in device order
     int32_t base;       // first qd index of the accumulator window
     int32_t width;      // window width; 0 = no window (global atomics)
     int32_t flags;      // kItemInterior: every window entry is owned by this item;
-                        // kItemDirect: every session owns its entries alone
+                        // kItemDirect: every session owns its entries alone;
+                        // kItemPacked: sessions stored as 16-byte records
     int32_t pad;
 };
 
@@ -54,16 +57,20 @@ struct TableArgs {
 };
 
 struct KArgs {
-    // Query-document indices, rank-major.  The sessions of window items come
-    // first in device order (positions [0, n_w)) and store 16-bit indices
-    // relative to their item's window; the others (positions [n_w, N)) store
-    // absolute 32-bit indices.
-    const uint16_t* pidx16;   // [kL][n_w]
+    // Sessions in device order, three formats:
+    //   [0, n_pk)   packed window items: one 16-byte record per session
+    //               (PackedSession below), indices relative to the window;
+    //   [n_pk, n_w) other window items: 16-bit window-relative indices,
+    //               rank-major;
+    //   [n_w, N)    the rest: absolute 32-bit indices, rank-major.
+    const uint32_t* packed;   // [n_pk][4]
+    const uint16_t* pidx16;   // [kL][n_w - n_pk]
     const int32_t* pidx;      // [kL][N - n_w] ((N - n_w) * 40 bytes < 2^32)
     const uint16_t* mask;     // [N] click bits (bit r = rank r)
     const uint8_t* len;       // [N] ranks shown, or nullptr
     int64_t N;
     int64_t n_w;
+    int64_t n_pk;
     const Item* items;
     int32_t n_items;
     int32_t mode;             // 0 segmented, 1 float atomics
@@ -604,29 +611,33 @@ template <> struct Shape<kCCM> { static constexpr int T = 1, K = 3, KACC = 6; };
 // address arithmetic per load) and hardware bounds checks (a prefetch past
 // the last session reads zeros).
 struct Stream {
-    __amdgpu_buffer_rsrc_t pidx16, pidx, mask, len;
-    uint32_t row16_bytes, row_bytes, n_w;
+    __amdgpu_buffer_rsrc_t packed, pidx16, pidx, mask, len;
+    uint32_t row16_bytes, row_bytes, n_w, n_pk;
 };
 
 __device__ __forceinline__ Stream make_stream(const KArgs& A) {
     constexpr int kRawDwordFlags = 0x00020000;   // raw 32-bit buffer, no format conversion
     Stream st;
     const uint32_t n_rest = (uint32_t)(A.N - A.n_w);
-    st.pidx16 = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx16, 0, (int)(uint32_t)(A.n_w * 20), kRawDwordFlags);
+    const uint32_t n_w16 = (uint32_t)(A.n_w - A.n_pk);
+    st.packed = __builtin_amdgcn_make_buffer_rsrc((void*)A.packed, 0, (int)(uint32_t)(A.n_pk * 16), kRawDwordFlags);
+    st.pidx16 = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx16, 0, (int)(n_w16 * 20u), kRawDwordFlags);
     st.pidx = __builtin_amdgcn_make_buffer_rsrc((void*)A.pidx, 0, (int)(n_rest * 40u), kRawDwordFlags);
     st.mask = __builtin_amdgcn_make_buffer_rsrc((void*)A.mask, 0, (int)(uint32_t)(A.N * 2), kRawDwordFlags);
     st.len = A.len ? __builtin_amdgcn_make_buffer_rsrc((void*)A.len, 0, (int)(uint32_t)A.N, kRawDwordFlags) : st.mask;
-    st.row16_bytes = (uint32_t)(A.n_w * 2);
+    st.row16_bytes = n_w16 * 2u;
     st.row_bytes = n_rest * 4u;
     st.n_w = (uint32_t)A.n_w;
+    st.n_pk = (uint32_t)A.n_pk;
     return st;
 }
 
 // Index of rank r of the session at device position o: a 16-bit window
-// offset for window items (WIN), else an absolute 32-bit index.
+// offset for window items stored rank-major (WIN), else an absolute 32-bit
+// index.
 template <bool WIN>
 __device__ __forceinline__ int load_index(const Stream& st, uint32_t o, int r) {
-    if (WIN) return (int)__builtin_amdgcn_raw_buffer_load_b16(st.pidx16, o * 2u, r * st.row16_bytes, 0);
+    if (WIN) return (int)__builtin_amdgcn_raw_buffer_load_b16(st.pidx16, (o - st.n_pk) * 2u, r * st.row16_bytes, 0);
     return (int)__builtin_amdgcn_raw_buffer_load_b32(st.pidx, (o - st.n_w) * 4u, r * st.row_bytes, 0);
 }
 
@@ -664,6 +675,30 @@ __device__ __forceinline__ void load_pidx_upto(const Stream& st, int s, int s_en
         pv[r] = 0;
         if (r < L) pv[r] = load_index<WIN>(st, o, r);
     }
+}
+
+// A packed session record (16 bytes, one load per session):
+//   bytes 0-9    offset of rank r's index from the session's base (ranks
+//                past the length hold 0),
+//   bytes 10-11  click bits,
+//   bytes 12-13  base: the smallest of the session's window-relative indices,
+//   byte  14     ranks shown, byte 15 unused.
+typedef unsigned int PackedSession __attribute__((ext_vector_type(4)));
+
+// The record of the session at position s (clamped as in load_session).
+__device__ __forceinline__ PackedSession load_packed(const Stream& st, int s, int s_end) {
+    const uint32_t o = (uint32_t)min(s, s_end - 1);
+    return __builtin_amdgcn_raw_buffer_load_b128(st.packed, o * 16u, 0, 0);
+}
+
+// Click bits, length and window-relative indices of a packed record.
+template <bool VARLEN>
+__device__ __forceinline__ void decode_packed(PackedSession v, uint32_t& m, int& L, int (&pv)[kL]) {
+    const int base = (int)(v[3] & 0xffffu);
+    m = v[2] >> 16;
+    L = VARLEN ? (int)((v[3] >> 16) & 0xffu) : kL;
+#pragma unroll
+    for (int r = 0; r < kL; r++) pv[r] = base + (int)((v[r >> 2] >> (8 * (r & 3))) & 0xffu);
 }
 
 // Wave-level sum of a 64-bit integer (wave64).

@@ -3,7 +3,8 @@
 // parameter-independent counts folded into the M-step, the work items
 // (query groups packed under LDS windows, slices of large groups, direct
 // single-session items), the device order of the sessions and their
-// rank-major index stream, and the minibatch lists of the gradient path.
+// records and rank-major index streams, and the minibatch lists of the
+// gradient path.
 // The device allocation and upload stay in mc_engine.hip.
 
 #include <algorithm>
@@ -136,6 +137,7 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
         int64_t count = 0;
         int32_t base = 0, width = 0, flags = 0;
         int kind = kKindWindow;
+        bool packed = false;    // window item stored as 16-byte session records
         int64_t gstart = 0;     // first session (query order) of the item's first group
     };
     std::vector<Spec> specs;
@@ -252,11 +254,36 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
             specs.push_back(c);
         }
     }
-    // Launch order: window items, then global items, then direct items; the
-    // largest items of each kind first (workgroups are dispatched in order, so
-    // long items do not end up in the tail).
+    // Packed window items: every session's indices lie within 256
+    // consecutive entries of the window (a query's entries are numbered
+    // contiguously, so this holds unless a query has more than 256 entries),
+    // and the session is stored as one 16-byte record: 8-bit offsets from its
+    // smallest index (segmented mode; the float-atomic kernels read the
+    // rank-major stream).
+    if (structured && m->packed && m->reduce_mode == MC_REDUCE_SEGMENTED) {
+        for (Spec& sp : specs) {
+            if (sp.kind != kKindWindow || sp.width > 65536) continue;
+            bool fits = true;
+            for (const auto& rg : sp.ranges)
+                for (int64_t s = rg.first; s < rg.second && fits; s++) {
+                    int32_t lo = INT32_MAX, hi = INT32_MIN;
+                    for (int r = 0; r < L[s]; r++) {
+                        lo = std::min(lo, pidx[s * kMaxSerp + r]);
+                        hi = std::max(hi, pidx[s * kMaxSerp + r]);
+                    }
+                    fits = hi - lo <= 255;
+                }
+            sp.packed = fits;
+            if (fits) sp.flags |= kItemPacked;
+        }
+    }
+    // Launch order: packed window items, other window items, global items,
+    // direct items; the largest items of each kind first (workgroups are
+    // dispatched in order, so long items do not end up in the tail).
     std::stable_sort(specs.begin(), specs.end(), [](const Spec& a, const Spec& b) {
-        return a.kind != b.kind ? a.kind < b.kind : a.count > b.count;
+        if (a.kind != b.kind) return a.kind < b.kind;
+        if (a.packed != b.packed) return a.packed;
+        return a.count > b.count;
     });
 
     // Device order: each item's sessions contiguous, sorted by last-click rank
@@ -265,7 +292,7 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
     std::vector<Item> items;
     std::vector<int64_t> order;
     order.reserve((size_t)n);
-    m->n_win_items = m->n_glob_items = m->n_dir_items = 0;
+    m->n_win_items = m->n_pk_items = m->n_glob_items = m->n_dir_items = 0;
     for (const Spec& sp : specs) {
         const int64_t j0 = (int64_t)order.size();
         for (const auto& rg : sp.ranges)
@@ -275,7 +302,10 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
                 return L[a] != L[b] ? L[a] < L[b] : lcr[a] < lcr[b];
             });
         items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, batch_of(sp.gstart)});
-        if (sp.kind == kKindWindow && structured) m->n_win_items++;
+        if (sp.kind == kKindWindow && structured) {
+            m->n_win_items++;
+            if (sp.packed) m->n_pk_items++;
+        }
         else if (sp.kind == kKindDirect) m->n_dir_items++;
         else m->n_glob_items++;
     }
@@ -348,32 +378,55 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
     // The UBM's fixed-point windows are followed by its examination slots.
     if (m->type == MC_UBM && m->reduce_mode == MC_REDUCE_SEGMENTED)
         m->lds_bytes = (m->lds_bytes + 7) / 8 * 8 + kUbmSlotBytes;
-    // ---- device layout: rank-major pidx, click masks, lengths ---------------
-    // Window items come first in device order.  Their sessions store 16-bit
-    // indices relative to the window base (the kernels address the LDS
-    // window with them): 20 bytes per session instead of 40.  The other
-    // sessions store absolute 32-bit indices.
+    // ---- device layout: session records, rank-major pidx, clicks, lengths --
+    // Window items come first in device order, packed ones first of all.
+    // Their sessions store indices relative to the window base (the kernels
+    // address the LDS window with them): 16-byte records (8-bit offsets from
+    // the session's smallest index, which is stored with the click bits and
+    // the length), else 16-bit indices, 20 bytes per session instead of 40.
+    // The other sessions store absolute 32-bit indices.  Click bits and
+    // lengths are also kept per position for every session.
     const int64_t n_w = m->n_win_items > 0 ? (int64_t)items[(size_t)m->n_win_items - 1].s1 : 0;
+    const int64_t n_pk = m->n_pk_items > 0 ? (int64_t)items[(size_t)m->n_pk_items - 1].s1 : 0;
+    const int64_t n_w16 = n_w - n_pk;
     const int64_t n_rest = n - n_w;
     MC_REQUIRE(n_rest * 40 < (int64_t)UINT32_MAX,
                "too many sessions outside window items (" + std::to_string(n_rest) +
                    ", at most 107,374,182 with 32-bit indices): use a larger MC_OPT_ITEM_WINDOW or fewer sessions");
+    MC_REQUIRE(n_pk * 16 < (int64_t)UINT32_MAX && n_w16 * 20 < (int64_t)UINT32_MAX, "internal: session stream size");
     m->n_w = n_w;
-    std::vector<uint16_t> h_pidx16((size_t)n_w * kMaxSerp);
+    m->n_pk = n_pk;
+    std::vector<uint32_t> h_packed((size_t)n_pk * 4);
+    std::vector<uint16_t> h_pidx16((size_t)n_w16 * kMaxSerp);
     std::vector<int32_t> h_pidx((size_t)n_rest * kMaxSerp);
     std::vector<uint16_t> h_mask((size_t)n);
     std::vector<uint8_t> h_len((size_t)n);
     for (size_t i = 0; i < items.size(); i++) {
         const Item& it = items[i];
         const bool win = (int)i < m->n_win_items;
+        const bool pk = (int)i < m->n_pk_items;
         MC_REQUIRE(!win || (it.width > 0 && it.width <= 65536 && it.s1 <= n_w), "internal: window item layout");
         MC_REQUIRE(win || it.s0 >= n_w, "internal: window items must come first");
+        MC_REQUIRE(pk == ((it.flags & kItemPacked) != 0) && (!pk || it.s1 <= n_pk) && (pk || it.s0 >= n_pk),
+                   "internal: packed items must come first");
         for (int64_t j = it.s0; j < it.s1; j++) {
             const int64_t s = order[j];
-            for (int r = 0; r < kMaxSerp; r++) {
-                const int32_t p = r < L[s] ? pidx[s * kMaxSerp + r] : 0;
-                if (win) h_pidx16[(size_t)r * n_w + j] = r < L[s] ? (uint16_t)(p - it.base) : 0;
-                else h_pidx[(size_t)r * n_rest + (j - n_w)] = p;
+            if (pk) {
+                int32_t lo = INT32_MAX;
+                for (int r = 0; r < L[s]; r++) lo = std::min(lo, pidx[s * kMaxSerp + r] - it.base);
+                uint32_t w[4] = {0u, 0u, (uint32_t)mask[s] << 16, (uint32_t)lo | ((uint32_t)L[s] << 16)};
+                for (int r = 0; r < L[s]; r++) {
+                    const int32_t off = pidx[s * kMaxSerp + r] - it.base - lo;
+                    MC_REQUIRE(off >= 0 && off <= 255 && lo >= 0 && lo <= 65535, "internal: packed session range");
+                    w[r >> 2] |= (uint32_t)off << (8 * (r & 3));
+                }
+                for (int k = 0; k < 4; k++) h_packed[(size_t)j * 4 + k] = w[k];
+            } else {
+                for (int r = 0; r < kMaxSerp; r++) {
+                    const int32_t p = r < L[s] ? pidx[s * kMaxSerp + r] : 0;
+                    if (win) h_pidx16[(size_t)r * n_w16 + (j - n_pk)] = r < L[s] ? (uint16_t)(p - it.base) : 0;
+                    else h_pidx[(size_t)r * n_rest + (j - n_w)] = p;
+                }
             }
             h_mask[j] = mask[s];
             h_len[j] = L[s];
@@ -387,6 +440,7 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
     h->items = std::move(items);
     h->shared_idx = std::move(shared_idx);
     h->lists = std::move(lists);
+    h->packed = std::move(h_packed);
     h->pidx16 = std::move(h_pidx16);
     h->pidx32 = std::move(h_pidx);
     h->mask = std::move(h_mask);

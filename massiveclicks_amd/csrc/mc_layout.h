@@ -17,7 +17,8 @@ struct HostLayout {
     std::vector<Item> items;         // work items in launch order
     std::vector<int32_t> shared_idx; // entries merged across items, ordered by minibatch
     std::vector<int32_t> lists;      // per minibatch: its window items, then its global items
-    std::vector<uint16_t> pidx16;    // [10][n_w] window-relative indices
+    std::vector<uint32_t> packed;    // [n_pk][4] packed session records
+    std::vector<uint16_t> pidx16;    // [10][n_w - n_pk] window-relative indices
     std::vector<int32_t> pidx32;     // [10][N - n_w] absolute indices
     std::vector<uint16_t> mask;      // [N] click bits
     std::vector<uint8_t> len;        // [N] ranks shown

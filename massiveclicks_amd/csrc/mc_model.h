@@ -34,21 +34,25 @@ struct mc_model {
     int block = 256;
     int direct = 0;
     int persistent = 1;
-    int persist_grid = 0;     // resident workgroups of the persistent window kernel
+    int packed = 1;           // MC_OPT_PACKED
+    int persist_grid[2] = {0, 0};   // resident workgroups of the persistent window kernel: [0] rank-major, [1] packed items
     // shape
     int T = 1, K = 10, KACC = 10;
     int64_t N = 0, n_qd = 0;
     bool varlen = false;
     bool initialized = false;
     // device buffers
-    uint16_t* d_pidx16 = nullptr;   // [10][n_w] window-relative indices of window items' sessions
+    uint32_t* d_packed = nullptr;   // [n_pk][4] 16-byte records of packed window items' sessions
+    uint16_t* d_pidx16 = nullptr;   // [10][n_w - n_pk] window-relative indices of the other window items' sessions
     int32_t* d_pidx = nullptr;      // [10][N - n_w] absolute indices of the other sessions
+    int64_t n_pk = 0;               // sessions in packed window items (device positions [0, n_pk))
     int64_t n_w = 0;                // sessions in window items (device positions [0, n_w))
     uint16_t* d_mask = nullptr;
     uint8_t* d_len = nullptr;
     mc::Item* d_items = nullptr;
     int n_items = 0;
-    int n_win_items = 0;      // items [0, n_win_items) have an LDS window,
+    int n_win_items = 0;      // items [0, n_win_items) have an LDS window (the first
+    int n_pk_items = 0;       //   n_pk_items of them packed),
     int n_glob_items = 0;     // then items merged with global atomics,
     int n_dir_items = 0;      // then direct items (single-session groups)
     int max_width = 0;

@@ -19,7 +19,7 @@
 namespace mc {
 namespace {
 
-constexpr int kNumOptions = 8;
+constexpr int kNumOptions = MC_N_OPTIONS;
 
 struct Trial {
     int model_type, device;
@@ -101,6 +101,8 @@ extern "C" mc_status mc_autotune(int model_type, int device, int64_t n, const in
         if (!best[MC_OPT_PERSISTENT])
             for (int64_t b : {256, 512})
                 if ((st = attempt(MC_OPT_BLOCK, b)) != MC_OK) return st;
+        // Session format of the window items: packed records or rank-major.
+        if ((st = attempt(MC_OPT_PACKED, best[MC_OPT_PACKED] ? 0 : 1)) != MC_OK) return st;
     } else {
         for (int64_t b : {256, 512})
             if ((st = attempt(MC_OPT_BLOCK, b)) != MC_OK) return st;

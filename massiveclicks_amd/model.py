@@ -21,7 +21,7 @@ MODEL_NAMES = {PBM: "PBM", CCM: "CCM", DBN: "DBN", UBM: "UBM"}
 MAX_SERP = 10
 
 (OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK, OPT_DIRECT, OPT_PERSISTENT,
- OPT_BATCHES) = range(8)
+ OPT_BATCHES, OPT_PACKED) = range(9)
 REDUCE_SEGMENTED, REDUCE_ATOMIC = 0, 1
 
 
@@ -130,7 +130,7 @@ class ClickModel:
     def __init__(self, model_type: int, device: int = 0, reduce_mode: int = REDUCE_SEGMENTED,
                  item_sessions: int | None = None, item_window: int | None = None,
                  sort_by_lcr: bool = True, block: int | None = None, direct: bool = False,
-                 persistent: bool = True, batches: int = 1):
+                 persistent: bool = True, batches: int = 1, packed: bool = True):
         self.model_type = int(model_type)
         self._h = C.c_void_p()
         L = _lib.lib()
@@ -140,6 +140,7 @@ class ClickModel:
         check(L.mc_model_set_option(self._h, OPT_DIRECT, int(bool(direct))))
         check(L.mc_model_set_option(self._h, OPT_PERSISTENT, int(bool(persistent))))
         check(L.mc_model_set_option(self._h, OPT_BATCHES, int(batches)))
+        check(L.mc_model_set_option(self._h, OPT_PACKED, int(bool(packed))))
         if item_sessions is not None:
             check(L.mc_model_set_option(self._h, OPT_ITEM_SESSIONS, int(item_sessions)))
         if item_window is not None:
@@ -353,17 +354,22 @@ class ClickModel:
         return {"n_items": a.value, "n_shared_entries": b.value, "max_item_sessions": c.value}
 
     def stream_info(self):
-        """Sessions of window items (16-bit window-relative indices) and the
-        device bytes of all session indices."""
+        """Sessions of window items (window-relative indices) and the device
+        bytes of all session indices (packed 16-byte records counted whole),
+        with the sessions per stored format."""
         a, b = C.c_int64(), C.c_int64()
-        check(_lib.lib().mc_get_stream_info(self._h, C.byref(a), C.byref(b)))
-        return {"n_window_sessions": a.value, "index_bytes": b.value}
+        L = _lib.lib()
+        check(L.mc_get_stream_info(self._h, C.byref(a), C.byref(b)))
+        p, w, g = C.c_int64(), C.c_int64(), C.c_int64()
+        check(L.mc_get_session_formats(self._h, C.byref(p), C.byref(w), C.byref(g)))
+        return {"n_window_sessions": a.value, "index_bytes": b.value, "n_packed": p.value, "n_window16": w.value,
+                "n_absolute": g.value}
 
 
 OPTION_NAMES = ("reduce_mode", "item_sessions", "item_window", "sort_by_lcr", "block", "direct", "persistent",
-                "batches")
+                "batches", "packed")
 DEFAULT_OPTIONS = {"reduce_mode": REDUCE_SEGMENTED, "item_sessions": 8192, "item_window": 1280, "sort_by_lcr": 1,
-                   "block": 256, "direct": 0, "persistent": 1, "batches": 1}
+                   "block": 256, "direct": 0, "persistent": 1, "batches": 1, "packed": 1}
 
 
 def autotune(model_type: int, train: SessionSet, n_qd: int, device: int = 0, iterations: int = 5,
@@ -396,6 +402,7 @@ def autotune(model_type: int, train: SessionSet, n_qd: int, device: int = 0, ite
     best["sort_by_lcr"] = bool(best["sort_by_lcr"])
     best["direct"] = bool(best["direct"])
     best["persistent"] = bool(best["persistent"])
+    best["packed"] = bool(best["packed"])
     if cache:
         known = {}
         if os.path.exists(cache):

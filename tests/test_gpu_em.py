@@ -100,15 +100,73 @@ def test_layout_options_keep_parity(model, opts):
     with run_gpu(model, ds, **opts) as cm:
         lay = cm.layout_info()
         si = cm.stream_info()
-        # Window items' sessions use 16-bit indices, the others 32-bit ones;
-        # tiny windows and direct items give a stream that mixes both.
+        # Window items' sessions use 16-byte records or 16-bit indices, the
+        # others 32-bit ones; tiny windows and direct items give a stream
+        # that mixes them.
         nw = si["n_window_sessions"]
-        assert 0 < nw <= n and si["index_bytes"] == 20 * nw + 40 * (n - nw), si
+        assert 0 < nw <= n and si["n_packed"] + si["n_window16"] == nw and si["n_absolute"] == n - nw, si
+        assert si["index_bytes"] == 16 * si["n_packed"] + 20 * si["n_window16"] + 40 * (n - nw), si
         if "item_window" in opts or "direct" in opts:
             assert nw < n, si
         for t, (onum, oden, oval) in enumerate(tabs):
             err = float(np.max(np.abs(cm.get_values(t) - oval)))
             assert err < TOL_PARAM, f"{opts} {lay} {si} table {t}: {err}"
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+def test_packed_records_equal_rank_major_indices(model):
+    """The same window items with sessions stored as 16-byte records
+    (MC_OPT_PACKED, every session of this workload fits) and as 16-bit
+    rank-major indices, one workgroup per item: identical device order and
+    arithmetic, so identical parameters bit for bit.  (The persistent
+    kernel's two forms may differ in resident workgroups, which changes the
+    order of the shared tables' per-lane float sums; they are gated against
+    the oracle by the other tests.)"""
+    ds, tabs, _ = workload(model)
+    print(f"case packed-vs-rank-major model={model}", flush=True)
+    with run_gpu(model, ds, persistent=False) as a, run_gpu(model, ds, persistent=False, packed=False) as b:
+        sa, sb = a.stream_info(), b.stream_info()
+        assert sa["n_packed"] == sa["n_window_sessions"] > 0 and sa["n_window16"] == 0, sa
+        assert sb["n_packed"] == 0 and sb["n_window16"] == sa["n_window_sessions"], sb
+        assert sa["index_bytes"] < sb["index_bytes"]
+        for t in range(len(tabs)):
+            np.testing.assert_array_equal(a.get_values(t), b.get_values(t))
+            np.testing.assert_array_equal(a.get_parameters(t)[0], b.get_parameters(t)[0])
+
+
+_wide = {}
+
+
+def wide_workload(model):
+    """Pools of 400 documents: the sessions of frequent queries show
+    documents more than 256 entries apart, so their window items keep 16-bit
+    rank-major indices while the rare queries' items are packed."""
+    if model not in _wide:
+        name = {PBM: "pbm", UBM: "ubm", DBN: "dbn", CCM: "ccm"}[model]
+        s = synth.generate(synth.SynthConfig(n_sessions=8000, n_queries=400, vocab=400000, pool=400, model=name,
+                                             seed=900 + model))
+        ds = oracle.from_sessions(s.query, s.docs, s.clicks, 0.2)
+        tr = ds["train"]
+        om = oracle.OracleModel(model, tr["query"], tr["pidx"], tr["clicks"], ds["n_qd"], accum_fp64=True)
+        om.iterate(ITERS)
+        _wide[model] = (ds, [om.get_table(t) for t in range(om.n_tables())])
+    return _wide[model]
+
+
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+@pytest.mark.parametrize("persistent", [True, False], ids=["persistent", "items"])
+def test_packed_and_rank_major_items_mix(model, persistent):
+    """A training set whose window items are partly packed and partly
+    rank-major (two persistent launches, or two item-kernel launches):
+    parameters within 1e-5 of the oracle after 50 iterations."""
+    ds, tabs = wide_workload(model)
+    print(f"case mixed-formats model={model} persistent={persistent}", flush=True)
+    with run_gpu(model, ds, persistent=persistent) as cm:
+        si = cm.stream_info()
+        assert si["n_packed"] > 0 and si["n_window16"] > 0, si
+        for t, (onum, oden, oval) in enumerate(tabs):
+            err = float(np.max(np.abs(cm.get_values(t) - oval)))
+            assert err < TOL_PARAM, f"{si} table {t}: max |value - oracle| = {err}"
 
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
