@@ -197,15 +197,22 @@ constexpr size_t kUbmSlotBytes = (size_t)kUbmSlots * kUbmCopies * 8;
 
 template <int T>
 struct EmitLds {            // fixed-point LDS window, relative indices
-    AccWindow<T> w;
-    int copy;                  // this lane's copy
+    char* base[T];             // this lane's copy of entry 0 of each table: &acc[slot(t, copy)]
+    int sh;                    // rsh + 3: entry p's copies start p << sh bytes further
     unsigned long long* slots; // UBM: this lane's copy of the examination slots (stride kUbmCopies)
     static constexpr bool kSlotsInLds = true;
-    __device__ __forceinline__ void add(int t, int, int p, float x) {
-        atomicAdd(&w.acc[w.slot(t, (p << w.rsh) + copy)], to_fix_unit(x));
+    __device__ __forceinline__ EmitLds(const AccWindow<T>& w, int copy, unsigned long long* lane_slots)
+        : sh(w.rsh + 3), slots(lane_slots) {
+#pragma unroll
+        for (int t = 0; t < T; t++) base[t] = (char*)&w.acc[w.slot(t, copy)];
     }
+    // One shift-add per emission: base[t] + (p << sh) is &acc[slot(t, (p << rsh) + copy)].
+    __device__ __forceinline__ unsigned long long* at(int t, int p) const {
+        return (unsigned long long*)(base[t] + (p << sh));
+    }
+    __device__ __forceinline__ void add(int t, int, int p, float x) { atomicAdd(at(t, p), to_fix_unit(x)); }
     __device__ __forceinline__ void add_wide(int t, int, int p, float x) {
-        atomicAdd(&w.acc[w.slot(t, (p << w.rsh) + copy)], (unsigned long long)to_fix(x));
+        atomicAdd(at(t, p), (unsigned long long)to_fix(x));
     }
     __device__ __forceinline__ void add_slot(int k, float x) { atomicAdd(&slots[k * kUbmCopies], to_fix_unit(x)); }
 };
