@@ -375,14 +375,15 @@ def main():
     # packed window item (offsets, base, clicks and length in one record),
     # 2 B per shown rank for sessions of the other window items (16-bit
     # window-relative indices), 4 B for the rest, with the mask (and length),
-    # and per entry an item owns the value in and out of each table plus its
-    # 2-byte packed counts.
+    # over the sessions the kernels visit (DBN and CCM skip the sessions
+    # without a click, whose counts are constants); and per entry an item owns
+    # the value in and out of each table plus its 2-byte packed counts.
     si = cm.stream_info()
     n_w = si["n_window_sessions"]
     n_pk = si["n_packed"]
     extra = 2 + (1 if batched else 0)
     layout_sess = (n_pk * 16 + (n_w - n_pk) * (2 * mean_len + extra) +
-                   (n_train - n_w) * (4 * mean_len + extra))
+                   (n_train - n_w) * (4 * mean_len + extra)) * (lay["n_em_sessions"] / n_train)
     layout_bytes = layout_sess + (8 * n_tab + 2) * (ds.n_qd - n_shared) if segmented else layout_sess
 
     # Held-out evaluation on the resident test set: the evaluation kernel's
@@ -447,7 +448,7 @@ def main():
                      "bytes_per_launch": item_bytes,
                      "layout_bytes_per_launch": layout_bytes,
                      "layout_frac": layout_bytes / (kt["ms_estep"] * 1e-3) / HBM_PEAK,
-                     "window_sessions": n_w, "packed_sessions": n_pk,
+                     "window_sessions": n_w, "packed_sessions": n_pk, "em_sessions": lay["n_em_sessions"],
                      "kernel_ms": kt["ms_estep"], "merge_kernel_ms": kt["ms_mstep"],
                      "iteration_bytes": iter_bytes,
                      "iteration_frac": iter_bytes / (ms_iter * 1e-3) / HBM_PEAK},

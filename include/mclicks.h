@@ -69,7 +69,14 @@ enum {
      * session as one 16-byte record (8-bit offsets from the session's
      * smallest index, the click bits and the length), read with one load;
      * 0 = every window item keeps 16-bit rank-major indices (default 1). */
-    MC_OPT_PACKED = 8
+    MC_OPT_PACKED = 8,
+    /* 1 = the EM kernels skip DBN and CCM sessions without a click (default
+     * 1).  By the reference's last-click convention (search.cu:160-173
+     * returns MAX_SERP when nothing is clicked) every count such a session
+     * contributes is parameter-independent; the counts are folded into the
+     * constant bases at init, so visiting them changes no result.  The
+     * sessions stay on the device for the gradient path.  0 = visit them. */
+    MC_OPT_SKIP_CONSTANT = 9
 };
 enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };
 
@@ -241,9 +248,10 @@ mc_status mc_get_memory_usage(const mc_model* m, size_t* bytes);
 mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, size_t* bytes);
 
 /* Work decomposition chosen at init: number of work items, number of per-qd
- * entries merged across items, and the largest item in sessions. */
+ * entries merged across items, the largest item in sessions, and the
+ * sessions the EM kernels visit per iteration (MC_OPT_SKIP_CONSTANT). */
 mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
-                             int64_t* max_item_sessions);
+                             int64_t* max_item_sessions, int64_t* n_em_sessions);
 
 /* Session stream chosen at init (no counterpart in the reference, which
  * keeps a 32-bit index per rank): sessions of window items store
@@ -258,7 +266,7 @@ mc_status mc_get_stream_info(const mc_model* m, int64_t* n_window_sessions, int6
 mc_status mc_get_session_formats(const mc_model* m, int64_t* n_packed, int64_t* n_window16, int64_t* n_absolute);
 
 /* Number of mc_model_set_option options (indices 0 .. MC_N_OPTIONS - 1). */
-#define MC_N_OPTIONS 9
+#define MC_N_OPTIONS 10
 
 /* Autotuner of the EM work decomposition; no counterpart in the reference
  * (whose launch shape is fixed: BLOCK_SIZE 128, one session per thread).

@@ -46,7 +46,7 @@ _SIGS = {
     "mc_evaluate_device": (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp]),
     "mc_get_memory_usage": (C.c_int, [vp, C.POINTER(C.c_size_t)]),
     "mc_compute_memory_footprint": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(C.c_size_t)]),
-    "mc_get_layout_info": (C.c_int, [vp, i64p, i64p, i64p]),
+    "mc_get_layout_info": (C.c_int, [vp, i64p, i64p, i64p, i64p]),
     "mc_get_stream_info": (C.c_int, [vp, i64p, i64p]),
     "mc_get_session_formats": (C.c_int, [vp, i64p, i64p, i64p]),
     "mc_get_model_type": (C.c_int, [vp, C.POINTER(C.c_int)]),

@@ -101,13 +101,13 @@ __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const
         // session ahead of its indices, which are read only up to it.
         uint32_t mn;
         int Ln;
-        load_head(st, it.s0 + (int)threadIdx.x + nthr, it.s1, mn, Ln);
-        for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
+        load_head(st, it.s0 + (int)threadIdx.x + nthr, it.e1, mn, Ln);
+        for (int s = it.s0 + (int)threadIdx.x; s < it.e1; s += nthr) {
             uint32_t m3;
             int L3;
             int pv2[kL];
-            load_head(st, s + 2 * nthr, it.s1, m3, L3);
-            load_pidx_upto<WIN>(st, s + nthr, it.s1, Ln, pv2);
+            load_head(st, s + 2 * nthr, it.e1, m3, L3);
+            load_pidx_upto<WIN>(st, s + nthr, it.e1, Ln, pv2);
             run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
             m = mn;
             L = Ln;
@@ -118,11 +118,11 @@ __device__ __forceinline__ void run_item(const Stream& st, const Item& it, const
         }
         return;
     }
-    for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
+    for (int s = it.s0 + (int)threadIdx.x; s < it.e1; s += nthr) {
         uint32_t m2;
         int L2;
         int pv2[kL];
-        load_session<VARLEN, WIN>(st, s + nthr, it.s1, m2, L2, pv2);
+        load_session<VARLEN, WIN>(st, s + nthr, it.e1, m2, L2, pv2);
         run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
         m = m2;
         L = L2;
@@ -138,8 +138,8 @@ template <int MODEL, bool VARLEN, class Src, class Emit>
 __device__ __forceinline__ void run_item_packed(const Stream& st, const Item& it, const Src& src, Emit& em,
                                                 const float* sh, float (&acc)[55], PackedSession cur) {
     const int nthr = blockDim.x;
-    for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
-        const PackedSession nxt = load_packed(st, s + nthr, it.s1);
+    for (int s = it.s0 + (int)threadIdx.x; s < it.e1; s += nthr) {
+        const PackedSession nxt = load_packed(st, s + nthr, it.e1);
         uint32_t m;
         int L;
         int pv[kL];
@@ -157,11 +157,11 @@ __device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st
                                                 int (&pv)[kL]) {
     constexpr int T = Shape<MODEL>::T;
     const int nthr = blockDim.x;
-    for (int s = it.s0 + (int)threadIdx.x; s < it.s1; s += nthr) {
+    for (int s = it.s0 + (int)threadIdx.x; s < it.e1; s += nthr) {
         uint32_t m2;
         int L2;
         int pv2[kL];
-        load_session<VARLEN, false>(st, s + nthr, it.s1, m2, L2, pv2);
+        load_session<VARLEN, false>(st, s + nthr, it.e1, m2, L2, pv2);
         EmitRegs em;
         em.clear();
         run_session<MODEL, VARLEN>(pv, L, m, src, em, sh, acc);
@@ -233,8 +233,8 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     int L0 = 0;
     int pv0[kL];
     PackedSession r0;
-    if (packed) r0 = load_packed(st, it.s0 + tid, it.s1);
-    else load_session<VARLEN, win>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    if (packed) r0 = load_packed(st, it.s0 + tid, it.e1);
+    else load_session<VARLEN, win>(st, it.s0 + tid, it.e1, m0, L0, pv0);
     for (int k = tid; k < K; k += nthr) sh[k] = A.sh_val_in[k];
     if (win) {
         for (int i = tid; i < W; i += nthr) {
@@ -352,17 +352,26 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 // in registers across items and are reduced once per workgroup.  Requires
 // blockDim.x == 256 and width <= 256 * PER.
 // Waves per SIMD the persistent kernel asks the compiler to fit: ten-rank DBN
-// and CCM go from 4 waves (108 and 114 VGPRs) to 5 (95 and 94 VGPRs, no
-// spills), which measured 1-3 % faster; the uneven-length UBM kernel goes
-// from 2 waves (181 VGPRs) to 3 (168 VGPRs, 4 dwords spilled), 22 % faster.
-// Ten-rank PBM fits 7 without a hint (the host caps it at 6 per CU: 7
-// measured slower), and ten-rank UBM at 5 waves (96 VGPRs, 40 bytes of
-// spills) measured 17 % slower than at 4.
-template <int MODEL, bool VARLEN>
-struct PersistWaves { static constexpr int value = (!VARLEN && (MODEL == kDBN || MODEL == kCCM)) ? 5 : (VARLEN && MODEL == kUBM) ? 3 : 1; };
+// goes from 4 waves (108 VGPRs) to 5 (89-95 VGPRs, no spills), 1-3 %
+// faster (at 6, 80 VGPRs with spills, it is 8 % slower and its 1280-entry
+// window's LDS allows only 5 workgroups per CU anyway); ten-rank CCM over
+// packed sessions runs 6 (80 VGPRs, 5 dwords spilled; one per-qd table, so
+// 6 windows fit), 2-3 % faster than 5 (its rank-major form would spill 15
+// dwords and keeps 5); the uneven-length UBM kernel goes from 2 waves (181 VGPRs)
+// to 3 (168 VGPRs, 4 dwords spilled), 22 % faster.  Ten-rank PBM fits 7
+// without a hint (the host caps it at 6 per CU: 7 measured slower), and
+// ten-rank UBM fits 6 (79 VGPRs) but its window plus examination slots
+// allow 5 workgroups per CU.
+template <int MODEL, bool VARLEN, bool PACKED>
+struct PersistWaves {
+    static constexpr int value = (!VARLEN && MODEL == kDBN)   ? 5
+                                 : (!VARLEN && MODEL == kCCM) ? (PACKED ? 6 : 5)
+                                 : (VARLEN && MODEL == kUBM)  ? 3
+                                                              : 1;
+};
 
 template <int MODEL, bool VARLEN, int PER, bool PACKED>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWaves<MODEL, VARLEN>::value)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWaves<MODEL, VARLEN, PACKED>::value)))
 em_window_persistent_kernel(KArgs A, int first, int n_win) {
     constexpr int T = Shape<MODEL>::T;
     constexpr int K = Shape<MODEL>::K;
@@ -388,7 +397,7 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
     auto item_of_round = [&](int k) { return k * G + ((k & 1) ? G - 1 - (int)blockIdx.x : (int)blockIdx.x); };
     int round = 0;
     int cur = item_of_round(0);
-    Item it = cur < n_win ? A.items[first + cur] : Item{0, 0, 0, 0, 0, 0};
+    Item it = cur < n_win ? A.items[first + cur] : Item{};
 
     // Window values of the current item, and its first session.
     float wv[PER][T];
@@ -402,8 +411,8 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
     int L0 = 0;
     int pv0[kL];
     PackedSession r0;
-    if (PACKED) r0 = load_packed(st, it.s0 + tid, it.s1);
-    else load_session<VARLEN, true>(st, it.s0 + tid, it.s1, m0, L0, pv0);
+    if (PACKED) r0 = load_packed(st, it.s0 + tid, it.e1);
+    else load_session<VARLEN, true>(st, it.s0 + tid, it.e1, m0, L0, pv0);
 
     float acc[55];
 #pragma unroll
@@ -428,7 +437,7 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
         }
         __syncthreads();
         const int nxt = item_of_round(round + 1);
-        const Item itn = nxt < n_win ? A.items[first + nxt] : Item{0, 0, 0, 0, 0, 0};
+        const Item itn = nxt < n_win ? A.items[first + nxt] : Item{};
         // In flight during the session loop: next window values, this
         // window's counts.
         // Ten-rank kernels load the next window after the session loop
@@ -461,8 +470,8 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
             }
         }
         // First session of the next item goes out before the flush.
-        if (PACKED) r0 = load_packed(st, itn.s0 + tid, itn.s1);
-        else load_session<VARLEN, true>(st, itn.s0 + tid, itn.s1, m0, L0, pv0);
+        if (PACKED) r0 = load_packed(st, itn.s0 + tid, itn.e1);
+        else load_session<VARLEN, true>(st, itn.s0 + tid, itn.e1, m0, L0, pv0);
         __syncthreads();
         const bool interior = (it.flags & kItemInterior) != 0;
 #pragma unroll
@@ -855,6 +864,9 @@ mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
         case MC_OPT_PACKED:
             m->packed = value != 0;
             break;
+        case MC_OPT_SKIP_CONSTANT:
+            m->skip_constant = value != 0;
+            break;
         case MC_OPT_BATCHES:
             MC_REQUIRE(value >= 1 && value <= 4096, "batches out of range [1, 4096]");
             m->n_batches = (int)value;
@@ -1240,11 +1252,12 @@ mc_status mc_compute_memory_footprint(int model_type, int64_t n, int64_t n_qd, s
 }
 
 mc_status mc_get_layout_info(const mc_model* m, int64_t* n_items, int64_t* n_shared_entries,
-                             int64_t* max_item_sessions) {
+                             int64_t* max_item_sessions, int64_t* n_em_sessions) {
     MC_REQUIRE(m && m->initialized, "model not initialised");
     if (n_items) *n_items = m->n_items;
     if (n_shared_entries) *n_shared_entries = m->n_shared;
     if (max_item_sessions) *max_item_sessions = m->max_item_sessions;
+    if (n_em_sessions) *n_em_sessions = m->n_em_sessions;
     return MC_OK;
 }
 

@@ -45,7 +45,11 @@ struct Item {
     int32_t flags;      // kItemInterior: every window entry is owned by this item;
                         // kItemDirect: every session owns its entries alone;
                         // kItemPacked: sessions stored as 16-byte records
-    int32_t pad;
+    int32_t pad;        // minibatch of the gradient path
+    int32_t e1;         // end of the sessions the EM kernels visit, s0 <= e1 <= s1: the
+                        // sessions [e1, s1) contribute only constant counts (DBN and
+                        // CCM sessions without a click) and are read by the gradient
+                        // path alone
 };
 
 struct TableArgs {

@@ -286,22 +286,40 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
         return a.count > b.count;
     });
 
-    // Device order: each item's sessions contiguous, sorted by last-click rank
-    // inside the item (wavefront-uniform chain lengths; the per-entry sums do
-    // not depend on the order).
+    // Device order: each item's sessions contiguous, sorted by length and
+    // last-click rank inside the item (wavefront-uniform chain lengths; the
+    // per-entry sums do not depend on the order).  DBN and CCM sessions
+    // without a click go last: by the reference's last-click convention
+    // (search.cu:160-173 returns MAX_SERP when nothing is clicked) every rank
+    // of such a session is "before the last click", so all its counts are the
+    // constants already folded into the bases above, and the EM kernels stop
+    // at the item's first such session (Item::e1).  The gradient path reads
+    // them (their click-free log-likelihood depends on the parameters).
+    // Direct items write their entries' values per session and visit all.
+    const bool constant_clickless = m->type == MC_DBN || m->type == MC_CCM;
     std::vector<Item> items;
     std::vector<int64_t> order;
     order.reserve((size_t)n);
     m->n_win_items = m->n_pk_items = m->n_glob_items = m->n_dir_items = 0;
+    m->n_em_sessions = 0;
     for (const Spec& sp : specs) {
         const int64_t j0 = (int64_t)order.size();
         for (const auto& rg : sp.ranges)
             for (int64_t s = rg.first; s < rg.second; s++) order.push_back(s);
+        auto clickless = [&](int64_t s) { return constant_clickless && mask[s] == 0; };
         if (m->sort_by_lcr)
             std::stable_sort(order.begin() + j0, order.end(), [&](int64_t a, int64_t b) {
+                if (clickless(a) != clickless(b)) return clickless(b);
                 return L[a] != L[b] ? L[a] < L[b] : lcr[a] < lcr[b];
             });
-        items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, batch_of(sp.gstart)});
+        else
+            std::stable_partition(order.begin() + j0, order.end(), [&](int64_t s) { return !clickless(s); });
+        int64_t e1 = (int64_t)order.size();
+        if (m->skip_constant && sp.kind != kKindDirect)
+            while (e1 > j0 && clickless(order[(size_t)e1 - 1])) e1--;
+        m->n_em_sessions += e1 - j0;
+        items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, batch_of(sp.gstart),
+                             (int32_t)e1});
         if (sp.kind == kKindWindow && structured) {
             m->n_win_items++;
             if (sp.packed) m->n_pk_items++;
@@ -315,7 +333,8 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
     {
         int64_t expect = 0;
         for (const Item& it : items) {
-            MC_REQUIRE(it.s0 == expect && it.s1 > it.s0 && it.s1 <= n, "internal: work items do not tile the sessions");
+            MC_REQUIRE(it.s0 == expect && it.s1 > it.s0 && it.s1 <= n && it.e1 >= it.s0 && it.e1 <= it.s1,
+                       "internal: work items do not tile the sessions");
             expect = it.s1;
             if (it.width > 0) {
                 MC_REQUIRE(it.base >= 0 && (int64_t)it.base + it.width <= n_qd && it.width <= W,

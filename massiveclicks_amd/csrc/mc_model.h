@@ -35,6 +35,7 @@ struct mc_model {
     int direct = 0;
     int persistent = 1;
     int packed = 1;           // MC_OPT_PACKED
+    int skip_constant = 1;    // MC_OPT_SKIP_CONSTANT
     int persist_grid[2] = {0, 0};   // resident workgroups of the persistent window kernel: [0] rank-major, [1] packed items
     // shape
     int T = 1, K = 10, KACC = 10;
@@ -47,6 +48,7 @@ struct mc_model {
     int32_t* d_pidx = nullptr;      // [10][N - n_w] absolute indices of the other sessions
     int64_t n_pk = 0;               // sessions in packed window items (device positions [0, n_pk))
     int64_t n_w = 0;                // sessions in window items (device positions [0, n_w))
+    int64_t n_em_sessions = 0;      // sessions the EM kernels visit (Item::e1)
     uint16_t* d_mask = nullptr;
     uint8_t* d_len = nullptr;
     mc::Item* d_items = nullptr;

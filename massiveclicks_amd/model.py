@@ -21,7 +21,7 @@ MODEL_NAMES = {PBM: "PBM", CCM: "CCM", DBN: "DBN", UBM: "UBM"}
 MAX_SERP = 10
 
 (OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK, OPT_DIRECT, OPT_PERSISTENT,
- OPT_BATCHES, OPT_PACKED) = range(9)
+ OPT_BATCHES, OPT_PACKED, OPT_SKIP_CONSTANT) = range(10)
 REDUCE_SEGMENTED, REDUCE_ATOMIC = 0, 1
 
 
@@ -130,7 +130,7 @@ class ClickModel:
     def __init__(self, model_type: int, device: int = 0, reduce_mode: int = REDUCE_SEGMENTED,
                  item_sessions: int | None = None, item_window: int | None = None,
                  sort_by_lcr: bool = True, block: int | None = None, direct: bool = False,
-                 persistent: bool = True, batches: int = 1, packed: bool = True):
+                 persistent: bool = True, batches: int = 1, packed: bool = True, skip_constant: bool = True):
         self.model_type = int(model_type)
         self._h = C.c_void_p()
         L = _lib.lib()
@@ -141,6 +141,7 @@ class ClickModel:
         check(L.mc_model_set_option(self._h, OPT_PERSISTENT, int(bool(persistent))))
         check(L.mc_model_set_option(self._h, OPT_BATCHES, int(batches)))
         check(L.mc_model_set_option(self._h, OPT_PACKED, int(bool(packed))))
+        check(L.mc_model_set_option(self._h, OPT_SKIP_CONSTANT, int(bool(skip_constant))))
         if item_sessions is not None:
             check(L.mc_model_set_option(self._h, OPT_ITEM_SESSIONS, int(item_sessions)))
         if item_window is not None:
@@ -349,9 +350,9 @@ class ClickModel:
         return b.value
 
     def layout_info(self):
-        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
-        check(_lib.lib().mc_get_layout_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return {"n_items": a.value, "n_shared_entries": b.value, "max_item_sessions": c.value}
+        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(_lib.lib().mc_get_layout_info(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"n_items": a.value, "n_shared_entries": b.value, "max_item_sessions": c.value, "n_em_sessions": d.value}
 
     def stream_info(self):
         """Sessions of window items (window-relative indices) and the device
@@ -367,9 +368,9 @@ class ClickModel:
 
 
 OPTION_NAMES = ("reduce_mode", "item_sessions", "item_window", "sort_by_lcr", "block", "direct", "persistent",
-                "batches", "packed")
+                "batches", "packed", "skip_constant")
 DEFAULT_OPTIONS = {"reduce_mode": REDUCE_SEGMENTED, "item_sessions": 8192, "item_window": 1280, "sort_by_lcr": 1,
-                   "block": 256, "direct": 0, "persistent": 1, "batches": 1, "packed": 1}
+                   "block": 256, "direct": 0, "persistent": 1, "batches": 1, "packed": 1, "skip_constant": 1}
 
 
 def autotune(model_type: int, train: SessionSet, n_qd: int, device: int = 0, iterations: int = 5,
@@ -403,6 +404,7 @@ def autotune(model_type: int, train: SessionSet, n_qd: int, device: int = 0, ite
     best["direct"] = bool(best["direct"])
     best["persistent"] = bool(best["persistent"])
     best["packed"] = bool(best["packed"])
+    best["skip_constant"] = bool(best["skip_constant"])
     if cache:
         known = {}
         if os.path.exists(cache):

@@ -392,6 +392,43 @@ def test_sessions_not_grouped_by_query_fall_back_to_atomics(model):
         assert e.value.status == 6
 
 
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+@pytest.mark.parametrize("uneven", [False, True], ids=["ten", "uneven"])
+@pytest.mark.parametrize("persistent", [True, False], ids=["persistent", "items"])
+def test_skipping_constant_sessions_changes_nothing(model, uneven, persistent):
+    """MC_OPT_SKIP_CONSTANT: DBN and CCM sessions without a click contribute
+    only constant counts (the reference's last-click rank of a click-free
+    session is MAX_SERP, so every rank counts as before the last click); the
+    EM kernels skip them, PBM and UBM visit every session.  The device order
+    does not depend on the option, so the parameters are identical bit for
+    bit with and without skipping."""
+    if uneven:
+        ds = varlen_workload(model)[0]
+        train, n_qd = ds.train, ds.n_qd
+        n = len(train.query)
+    else:
+        d = workload(model)[0]
+        train, n_qd = sset(d["train"]), d["n_qd"]
+        n = len(d["train"]["query"])
+    clickless = int((np.asarray(train.clicks).reshape(n, -1).sum(1) == 0).sum())
+    print(f"case skip-constant model={model} uneven={uneven} persistent={persistent} clickless={clickless}", flush=True)
+    with ClickModel(model, device=0, persistent=persistent) as a, \
+            ClickModel(model, device=0, persistent=persistent, skip_constant=False) as b:
+        for cm in (a, b):
+            if uneven:
+                cm.init_from_dataset(ds)
+            else:
+                cm.init_parameters(train, n_qd)
+            cm.em(ITERS)
+        visited = a.layout_info()["n_em_sessions"]
+        assert b.layout_info()["n_em_sessions"] == n
+        assert clickless > 0
+        assert visited == (n - clickless if model in (DBN, CCM) else n), (visited, n, clickless)
+        for t in range(a.n_tables()):
+            np.testing.assert_array_equal(a.get_values(t), b.get_values(t))
+            np.testing.assert_array_equal(a.get_parameters(t)[0], b.get_parameters(t)[0])
+
+
 VARLEN_LAYOUTS = {
     "items-b256": dict(persistent=False, block=256),
     "items-b512": dict(persistent=False, block=512),
