@@ -601,6 +601,7 @@ void release(mc_model* m) {
     for (void* p : m->allocations) (void)hipFree(p);
     m->allocations.clear();
     m->mem_bytes = 0;
+    m->d_test_packed = nullptr;
     m->d_test_pidx = nullptr;
     m->d_test_mask = nullptr;
     m->d_test_len = nullptr;

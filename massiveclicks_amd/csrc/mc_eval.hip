@@ -5,15 +5,18 @@
 // and the log-likelihood and perplexity totals (evaluation.cpp:25-66) in
 // FP64.
 //
-// Two session layouts feed the same per-session arithmetic:
+// Three session layouts feed the same per-session arithmetic:
 //   * the caller's arrays, session-major ([n][10] int32 pair indices and
 //     [n][10] click bytes), uploaded per call (mc_evaluate,
 //     mc_serp_probability);
 //   * a held-out set kept resident on the device (mc_set_test_set): sessions
 //     reordered so that the sessions of one query are adjacent (their value
-//     gathers then share cache lines), stored rank-major ([10][n] int32 pair
-//     indices, 16-bit click masks), with the permutation back to the
-//     caller's order for the per-session outputs.
+//     gathers then share cache lines), with the permutation back to the
+//     caller's order for the per-session outputs, stored as one 16-byte
+//     record per session when every session's seen pairs lie within 255
+//     consecutive indices (pairs are numbered per query in contiguous
+//     ranges), else rank-major ([10][n] int32 pair indices, 16-bit click
+//     masks).
 
 #include <hip/hip_runtime.h>
 
@@ -34,7 +37,8 @@ struct EvalArgs {
     // Session-major layout.
     const int32_t* pidx;     // [n][10]
     const uint8_t* clicks;   // [n][10]
-    // Rank-major layout.
+    // Resident layouts: packed records or rank-major.
+    const uint4* packed;     // [n]: offsets, clicks and length, base (load_packed_test)
     const int32_t* rpidx;    // [10][n]
     const uint16_t* mask;    // [n]
     const int32_t* perm;     // [n]: caller's index of the session at position i, or null
@@ -88,6 +92,26 @@ __device__ __forceinline__ void load_rank_major(const EvalArgs& E, int64_t i, in
 #pragma unroll
     for (int r = 0; r < kL; r++) p[r] = E.rpidx[r * E.n + i];
 }
+
+// Packed test record: bytes 0-9 the offset of rank r's pair from the base
+// (255: a pair unseen in training, or a rank past the length), bits 16-25
+// of the third word the click bits and bits 26-29 the length, the fourth
+// word the base (the session's smallest seen pair index).  One 16-byte load
+// per session.
+constexpr uint32_t kTestUnseen = 255u;
+__device__ __forceinline__ void decode_packed_test(uint4 v, int& L, int (&p)[kL], uint32_t& c) {
+    c = (v.z >> 16) & 0x3ffu;
+    L = min(max((int)((v.z >> 26) & 0xfu), 1), kL);
+    const uint32_t w[3] = {v.x, v.y, v.z};
+#pragma unroll
+    for (int r = 0; r < kL; r++) {
+        const uint32_t b = (w[r >> 2] >> (8 * (r & 3))) & 0xffu;
+        p[r] = b == kTestUnseen ? -1 : (int)(v.w + b);
+    }
+}
+
+// Session layouts of eval_kernel.
+enum { kEvalSessionMajor = 0, kEvalRankMajor = 1, kEvalPacked = 2 };
 
 // The per-session arithmetic, the reference's operation for operation in
 // FP32: log conditional click probabilities lp, full click probabilities
@@ -218,7 +242,7 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // session counts come from wavefront ballots (uniform values, no vector
 // registers).  OUT: per-session outputs requested (separate instantiation,
 // so that the common case keeps fewer values live).
-template <int MODEL, bool RANK_MAJOR, bool OUT>
+template <int MODEL, int LAYOUT, bool OUT>
 __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
     __shared__ double red[21][4];
     const int tid = threadIdx.x;
@@ -232,21 +256,31 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
     // session is evaluated.
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + tid;
+    // Packed records stay raw (4 registers) until their session is reached.
     int Ln = kL;
     int pn[kL];
     uint32_t cn = 0;
+    uint4 rn = make_uint4(0u, 0u, 0u, 0u);
     auto load = [&](int64_t j) {
+        if (LAYOUT == kEvalPacked) {
+            rn = E.packed[j];
+            return;
+        }
         Ln = E.len ? min(max((int)E.len[j], 1), kL) : kL;
-        if (RANK_MAJOR) load_rank_major(E, j, pn, cn);
+        if (LAYOUT == kEvalRankMajor) load_rank_major(E, j, pn, cn);
         else load_session_major(E, j, pn, cn);
     };
     if (i < E.n) load(i);
     for (; i < E.n; i += stride) {
-        const int L = Ln;
-        const uint32_t c = cn;
+        int L = Ln;
+        uint32_t c = cn;
         int p[kL];
+        if (LAYOUT == kEvalPacked) {
+            decode_packed_test(rn, L, p, c);
+        } else {
 #pragma unroll
-        for (int r = 0; r < kL; r++) p[r] = pn[r];
+            for (int r = 0; r < kL; r++) p[r] = pn[r];
+        }
         if (i + stride < E.n) load(i + stride);
         float lp[kL], fp[kL], sp[kL];
         eval_session<MODEL>(E, L, p, c, lp, fp, sp);
@@ -256,7 +290,7 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
             if (r < L) acc = acc + lp[r];
         const float s = acc / L;                 // evaluation.cpp:36
         v[0] += (double)s;
-        const int64_t o = (OUT && RANK_MAJOR && E.perm) ? (int64_t)E.perm[i] : i;
+        const int64_t o = (OUT && LAYOUT != kEvalSessionMajor && E.perm) ? (int64_t)E.perm[i] : i;
         if (OUT && E.llh) E.llh[o] = s;
 #pragma unroll
         for (int r = 0; r < kL; r++) {
@@ -341,20 +375,20 @@ double* eval_scratch(int device) {
     return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
 }
 
-template <bool RANK_MAJOR, bool OUT>
+template <int LAYOUT, bool OUT>
 void launch_eval_out(int type, const EvalArgs& E, int64_t nb, hipStream_t s) {
     switch (type) {
-        case MC_PBM: hipLaunchKernelGGL((eval_kernel<kPBM, RANK_MAJOR, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
-        case MC_UBM: hipLaunchKernelGGL((eval_kernel<kUBM, RANK_MAJOR, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
-        case MC_DBN: hipLaunchKernelGGL((eval_kernel<kDBN, RANK_MAJOR, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
-        default: hipLaunchKernelGGL((eval_kernel<kCCM, RANK_MAJOR, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+        case MC_PBM: hipLaunchKernelGGL((eval_kernel<kPBM, LAYOUT, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+        case MC_UBM: hipLaunchKernelGGL((eval_kernel<kUBM, LAYOUT, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+        case MC_DBN: hipLaunchKernelGGL((eval_kernel<kDBN, LAYOUT, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
+        default: hipLaunchKernelGGL((eval_kernel<kCCM, LAYOUT, OUT>), dim3((unsigned)nb), dim3(256), 0, s, E); break;
     }
 }
 
-template <bool RANK_MAJOR>
+template <int LAYOUT>
 void launch_eval(int type, const EvalArgs& E, int64_t nb, hipStream_t s) {
-    if (E.logp || E.full || E.llh || E.prob) launch_eval_out<RANK_MAJOR, true>(type, E, nb, s);
-    else launch_eval_out<RANK_MAJOR, false>(type, E, nb, s);
+    if (E.logp || E.full || E.llh || E.prob) launch_eval_out<LAYOUT, true>(type, E, nb, s);
+    else launch_eval_out<LAYOUT, false>(type, E, nb, s);
 }
 
 EvalArgs model_eval_args(const mc_model* m) {
@@ -434,7 +468,7 @@ mc_status evaluate_host_arrays(mc_model* m, int64_t n, const int32_t* pidx, cons
     E.llh = d_llh;
     E.prob = d_pr;
     E.partial = d_part;
-    launch_eval<false>(m->type, E, nb, m->stream);
+    launch_eval<kEvalSessionMajor>(m->type, E, nb, m->stream);
     chk(hipGetLastError());
     hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, d_part, (int)nb, d_out);
     chk(hipGetLastError());
@@ -499,7 +533,7 @@ mc_status mc_evaluate_device(mc_model* m, int64_t n, const int32_t* pidx, const 
     E.full = full;
     E.llh = llh;
     E.partial = part;
-    launch_eval<false>(m->type, E, nb, m->stream);
+    launch_eval<kEvalSessionMajor>(m->type, E, nb, m->stream);
     MC_HIP(hipGetLastError());
     hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, (const double*)part, (int)nb, totals);
     MC_HIP(hipGetLastError());
@@ -532,23 +566,47 @@ mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uin
     for (int64_t k = 0; k <= n_qd; k++) start[(size_t)k + 1] += start[(size_t)k];
     std::vector<int32_t> perm((size_t)n);
     for (int64_t i = 0; i < n; i++) perm[(size_t)start[(size_t)key[(size_t)i]]++] = (int32_t)i;
-    std::vector<int32_t> h_p((size_t)n * kMaxSerp);
-    std::vector<uint16_t> h_m((size_t)n);
-    std::vector<uint8_t> h_l(len ? (size_t)n : 0);
+    // Packed records when every session's seen pairs span at most 255
+    // indices (the base is the smallest, key[i]); else rank-major.
+    bool packable = true;
+    for (int64_t i = 0; i < n && packable; i++) {
+        const int l = len ? len[i] : kMaxSerp;
+        for (int r = 0; r < l; r++) {
+            const int32_t p = pidx[i * kMaxSerp + r];
+            if (p >= 0 && p < n_qd && (int64_t)p - key[(size_t)i] >= (int64_t)kTestUnseen) packable = false;
+        }
+    }
+    std::vector<uint32_t> h_pk(packable ? (size_t)n * 4 : 0);
+    std::vector<int32_t> h_p(packable ? 0 : (size_t)n * kMaxSerp);
+    std::vector<uint16_t> h_m(packable ? 0 : (size_t)n);
+    std::vector<uint8_t> h_l(len && !packable ? (size_t)n : 0);
     for (int64_t j = 0; j < n; j++) {
         const int64_t i = perm[(size_t)j];
         const int l = len ? len[i] : kMaxSerp;
         uint16_t mk = 0;
-        for (int r = 0; r < kMaxSerp; r++) {
-            h_p[(size_t)r * n + j] = r < l ? pidx[i * kMaxSerp + r] : -1;
-            if (r < l && clicks[i * kMaxSerp + r]) mk |= (uint16_t)(1u << r);
+        for (int r = 0; r < l; r++)
+            if (clicks[i * kMaxSerp + r]) mk |= (uint16_t)(1u << r);
+        if (packable) {
+            const uint32_t base = key[(size_t)i] < n_qd ? (uint32_t)key[(size_t)i] : 0u;
+            uint8_t b[12];
+            for (int r = 0; r < kMaxSerp; r++) {
+                const int32_t p = r < l ? pidx[i * kMaxSerp + r] : -1;
+                b[r] = (p >= 0 && p < n_qd) ? (uint8_t)(p - (int32_t)base) : (uint8_t)kTestUnseen;
+            }
+            const uint32_t meta = (uint32_t)mk | ((uint32_t)l << 10);
+            b[10] = (uint8_t)(meta & 0xffu);
+            b[11] = (uint8_t)(meta >> 8);
+            std::memcpy(&h_pk[(size_t)j * 4], b, 12);
+            h_pk[(size_t)j * 4 + 3] = base;
+        } else {
+            for (int r = 0; r < kMaxSerp; r++) h_p[(size_t)r * n + j] = r < l ? pidx[i * kMaxSerp + r] : -1;
+            h_m[(size_t)j] = mk;
+            if (len) h_l[(size_t)j] = (uint8_t)l;
         }
-        h_m[(size_t)j] = mk;
-        if (len) h_l[(size_t)j] = (uint8_t)l;
     }
     // Buffers of an earlier test set are replaced.
-    for (void* p : {(void*)m->d_test_pidx, (void*)m->d_test_mask, (void*)m->d_test_len, (void*)m->d_test_perm,
-                    (void*)m->d_test_part}) {
+    for (void* p : {(void*)m->d_test_packed, (void*)m->d_test_pidx, (void*)m->d_test_mask, (void*)m->d_test_len,
+                    (void*)m->d_test_perm, (void*)m->d_test_part}) {
         if (!p) continue;
         auto it = std::find(m->allocations.begin(), m->allocations.end(), p);
         if (it != m->allocations.end()) m->allocations.erase(it);
@@ -556,6 +614,7 @@ mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uin
     }
     m->mem_bytes -= m->test_bytes;
     m->test_bytes = 0;
+    m->d_test_packed = nullptr;
     m->d_test_pidx = nullptr;
     m->d_test_mask = nullptr;
     m->d_test_len = nullptr;
@@ -564,15 +623,23 @@ mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uin
     m->n_test = 0;
     const size_t before = m->mem_bytes;
     const int64_t nb = eval_blocks(n);
-    MC_TRY(dalloc(m, &m->d_test_pidx, h_p.size() * 4));
-    MC_TRY(dalloc(m, &m->d_test_mask, h_m.size() * 2));
-    if (len) MC_TRY(dalloc(m, &m->d_test_len, h_l.size()));
+    if (packable) {
+        MC_TRY(dalloc(m, &m->d_test_packed, h_pk.size() * 4));
+    } else {
+        MC_TRY(dalloc(m, &m->d_test_pidx, h_p.size() * 4));
+        MC_TRY(dalloc(m, &m->d_test_mask, h_m.size() * 2));
+        if (len) MC_TRY(dalloc(m, &m->d_test_len, h_l.size()));
+    }
     MC_TRY(dalloc(m, &m->d_test_perm, perm.size() * 4));
     MC_TRY(dalloc(m, &m->d_test_part, ((size_t)nb * 21 + 21) * 8));
     m->test_bytes = m->mem_bytes - before;
-    MC_HIP(hipMemcpyAsync(m->d_test_pidx, h_p.data(), h_p.size() * 4, hipMemcpyHostToDevice, m->stream));
-    MC_HIP(hipMemcpyAsync(m->d_test_mask, h_m.data(), h_m.size() * 2, hipMemcpyHostToDevice, m->stream));
-    if (len) MC_HIP(hipMemcpyAsync(m->d_test_len, h_l.data(), h_l.size(), hipMemcpyHostToDevice, m->stream));
+    if (packable) {
+        MC_HIP(hipMemcpyAsync(m->d_test_packed, h_pk.data(), h_pk.size() * 4, hipMemcpyHostToDevice, m->stream));
+    } else {
+        MC_HIP(hipMemcpyAsync(m->d_test_pidx, h_p.data(), h_p.size() * 4, hipMemcpyHostToDevice, m->stream));
+        MC_HIP(hipMemcpyAsync(m->d_test_mask, h_m.data(), h_m.size() * 2, hipMemcpyHostToDevice, m->stream));
+        if (len) MC_HIP(hipMemcpyAsync(m->d_test_len, h_l.data(), h_l.size(), hipMemcpyHostToDevice, m->stream));
+    }
     MC_HIP(hipMemcpyAsync(m->d_test_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipStreamSynchronize(m->stream));
     m->n_test = n;
@@ -603,6 +670,7 @@ mc_status mc_evaluate_test_set(mc_model* m, float* logp, float* full, float* llh
         return MC_ERR_OOM;
     }
     EvalArgs E = model_eval_args(m);
+    E.packed = reinterpret_cast<const uint4*>(m->d_test_packed);
     E.rpidx = m->d_test_pidx;
     E.mask = m->d_test_mask;
     E.perm = m->d_test_perm;
@@ -621,7 +689,8 @@ mc_status mc_evaluate_test_set(mc_model* m, float* logp, float* full, float* llh
         chk(hipEventCreate(&ev[1]));
         chk(hipEventRecord(ev[0], m->stream));
     }
-    launch_eval<true>(m->type, E, nb, m->stream);
+    if (m->d_test_packed) launch_eval<kEvalPacked>(m->type, E, nb, m->stream);
+    else launch_eval<kEvalRankMajor>(m->type, E, nb, m->stream);
     chk(hipGetLastError());
     if (kernel_ms) chk(hipEventRecord(ev[1], m->stream));
     hipLaunchKernelGGL(eval_reduce_kernel, dim3(21), dim3(256), 0, m->stream, (const double*)m->d_test_part, (int)nb,

@@ -110,6 +110,7 @@ struct mc_model {
     int adam_graph_first = -1;
     bool value_only = false;                   // tables hold values only (numerators stale)
     // ---- resident held-out set (mc_eval.hip) --------------------------------
+    uint32_t* d_test_packed = nullptr;         // [n_test][4] packed test sessions, or null (rank-major set)
     int32_t* d_test_pidx = nullptr;            // [10][n_test] pair indices (-1 past the length / unseen)
     uint16_t* d_test_mask = nullptr;           // [n_test] click bits
     uint8_t* d_test_len = nullptr;             // [n_test] or null (10 ranks)

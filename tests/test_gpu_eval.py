@@ -61,6 +61,42 @@ def test_resident_test_set_matches_oracle(model, uneven):
     assert ev["llh_sum"] == pytest.approx(ref["llh_sum"], rel=1e-12)
 
 
+@pytest.mark.parametrize("pool", [20, 400], ids=["packed", "rank-major"])
+def test_resident_layout_follows_the_pair_spans(pool):
+    """The resident set is stored as 16-byte records when every test
+    session's seen pairs lie within 255 consecutive indices (pools of 20
+    documents), else rank-major (pools of 400: a session can show pairs
+    further apart); both match the oracle."""
+    s = synth.generate(synth.SynthConfig(n_sessions=6000, n_queries=100, vocab=400000, pool=pool, model="dbn",
+                                         seed=777 + pool))
+    ds = Dataset.from_sessions(s.query, s.docs, s.clicks, 0.2)
+    te = ds.test
+    p = te.pidx.astype(np.int64)
+    seen = p >= 0
+    lo = np.where(seen, p, np.iinfo(np.int64).max).min(1)
+    span = np.where(seen, p - lo[:, None], 0).max(1)
+    packable = bool((span < 255).all())
+    assert packable == (pool == 20), span.max()
+    with ClickModel(DBN) as cm:
+        cm.init_from_dataset(ds)
+        cm.em(10)
+        tr = ds.train
+        om = oracle.OracleModel(DBN, tr.query, tr.pidx, tr.clicks, ds.n_qd)
+        for t in range(cm.n_tables()):
+            om.set_table(t, *cm.get_parameters(t))
+        before = cm.get_memory_usage()
+        cm.set_test_set(te)
+        per_session = (cm.get_memory_usage() - before) / len(te)
+        print(f"case resident layout pool={pool} n_test={len(te)} bytes/session={per_session:.1f}", flush=True)
+        assert (per_session < 32) == packable, per_session
+        ev = cm.evaluate_test_set(per_session=True)
+        oev = om.evaluate(te.pidx, te.clicks, te.length)
+        np.testing.assert_allclose(ev["llh"], oev["llh"], rtol=2e-6, atol=1e-7)
+        np.testing.assert_allclose(ev["logp"], oev["logp"], rtol=2e-6, atol=1e-7)
+        assert abs(ev["llh_sum"] - oev["llh_sum_f64"]) <= 1e-7 * abs(oev["llh_sum_f64"])
+        np.testing.assert_array_equal(ev["rank_count"], oev["rank_count"])
+
+
 def test_resident_test_set_is_replaced_and_dropped_by_init():
     """A second mc_set_test_set replaces the first (here a smaller subset in
     another order); mc_init_parameters drops the set and its memory."""
