@@ -116,18 +116,24 @@ enum { kEvalSessionMajor = 0, kEvalRankMajor = 1, kEvalPacked = 2 };
 // The per-session arithmetic, the reference's operation for operation in
 // FP32: log conditional click probabilities lp, full click probabilities
 // fp and the SERP click probabilities sp of the first L ranks; click bits cm.
+// The session's parameter values: attractiveness a and (DBN) satisfaction
+// sv of its first L ranks.
 template <int MODEL>
-__device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const int (&p)[kL], uint32_t cm,
-                                             float (&lp)[kL], float (&fp)[kL], float (&sp)[kL]) {
-    bool c[kL];
-#pragma unroll
-    for (int r = 0; r < kL; r++) c[r] = (cm >> r) & 1u;
-    float a[kL], sv[kL];
+__device__ __forceinline__ void gather_values(const EvalArgs& E, int L, const int (&p)[kL], float (&a)[kL],
+                                              float (&sv)[kL]) {
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         a[r] = r < L ? pv_or_default(E.val0, p[r], E.n_qd) : 0.5f;
         sv[r] = (MODEL == kDBN && r < L) ? pv_or_default(E.val1, p[r], E.n_qd) : 0.5f;
     }
+}
+
+template <int MODEL>
+__device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const float (&a)[kL], const float (&sv)[kL],
+                                             uint32_t cm, float (&lp)[kL], float (&fp)[kL], float (&sp)[kL]) {
+    bool c[kL];
+#pragma unroll
+    for (int r = 0; r < kL; r++) c[r] = (cm >> r) & 1u;
     if (MODEL == kPBM) {
 #pragma unroll
         for (int r = 0; r < kL; r++) {
@@ -270,20 +276,27 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
         if (LAYOUT == kEvalRankMajor) load_rank_major(E, j, pn, cn);
         else load_session_major(E, j, pn, cn);
     };
-    if (i < E.n) load(i);
-    for (; i < E.n; i += stride) {
-        int L = Ln;
-        uint32_t c = cn;
+    auto take = [&](int& L, uint32_t& c, float (&a)[kL], float (&sv)[kL]) {
         int p[kL];
+        L = Ln;
+        c = cn;
         if (LAYOUT == kEvalPacked) {
             decode_packed_test(rn, L, p, c);
         } else {
 #pragma unroll
             for (int r = 0; r < kL; r++) p[r] = pn[r];
         }
+        gather_values<MODEL>(E, L, p, a, sv);
+    };
+    if (i < E.n) load(i);
+    for (; i < E.n; i += stride) {
+        int L;
+        uint32_t c;
+        float a[kL], sv[kL];
+        take(L, c, a, sv);
         if (i + stride < E.n) load(i + stride);
         float lp[kL], fp[kL], sp[kL];
-        eval_session<MODEL>(E, L, p, c, lp, fp, sp);
+        eval_session<MODEL>(E, L, a, sv, c, lp, fp, sp);
         float acc = 0.f;
 #pragma unroll
         for (int r = 0; r < kL; r++)
