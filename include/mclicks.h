@@ -76,7 +76,13 @@ enum {
      * contributes is parameter-independent; the counts are folded into the
      * constant bases at init, so visiting them changes no result.  The
      * sessions stay on the device for the gradient path.  0 = visit them. */
-    MC_OPT_SKIP_CONSTANT = 9
+    MC_OPT_SKIP_CONSTANT = 9,
+    /* 1 = when the persistent window kernel is an iteration's only E-step
+     * launch, it also runs the M-step: the workgroup that adds the last
+     * slice of a query group split over several work items finalises the
+     * group's entries, and the last workgroup to finish the shared tables,
+     * so no merge kernel is launched (default 1).  Same results. */
+    MC_OPT_FUSED_MERGE = 10
 };
 enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };
 
@@ -266,7 +272,7 @@ mc_status mc_get_stream_info(const mc_model* m, int64_t* n_window_sessions, int6
 mc_status mc_get_session_formats(const mc_model* m, int64_t* n_packed, int64_t* n_window16, int64_t* n_absolute);
 
 /* Number of mc_model_set_option options (indices 0 .. MC_N_OPTIONS - 1). */
-#define MC_N_OPTIONS 10
+#define MC_N_OPTIONS 11
 
 /* Autotuner of the EM work decomposition; no counterpart in the reference
  * (whose launch shape is fixed: BLOCK_SIZE 128, one session per thread).

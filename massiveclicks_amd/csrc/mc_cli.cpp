@@ -211,7 +211,7 @@ int main(int argc, char** argv) {
 
         // ---- EM (parallel_em.cu:349-609) ------------------------------------
         const auto t_em = std::chrono::steady_clock::now();
-        int64_t opts[MC_N_OPTIONS] = {config.reduce_mode, 8192, 1280, 1, 256, 0, 1, 1, 1, 1};
+        int64_t opts[MC_N_OPTIONS] = {config.reduce_mode, 8192, 1280, 1, 256, 0, 1, 1, 1, 1, 1};
         if (config.autotune) {
             const mclicks::Sessions tr = ds.train();
             double ms = 0.0;

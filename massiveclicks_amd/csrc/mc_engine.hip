@@ -337,6 +337,56 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     }
 }
 
+// Cross-workgroup hand-over inside one launch (fused M-step).  The data
+// handed over (fixed-point sums, counters) is only ever written with
+// device-scope atomics, which are performed at the device's coherence point;
+// so the writer only has to wait until its atomics have completed before it
+// bumps a counter (an agent-scope release fence would also write back the
+// XCD's L2 for plain stores, which nothing here needs and which cost 4x the
+// kernel's time), and the reader reads with device-coherent loads.
+__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+__device__ __forceinline__ unsigned long long load_acc(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// M-step of the shared (rank-indexed) tables from their fixed-point sums and
+// FP64 bases, threads tid < K; then the accumulators are cleared for the next
+// iteration.  COHERENT: the sums were added during this launch.
+template <bool COHERENT>
+__device__ __forceinline__ void finalize_shared(const KArgs& A, int tid, int nthr) {
+    const int k = tid;
+    if (k < A.K) {
+        const int an = A.sh_acc_map[2 * k], ad = A.sh_acc_map[2 * k + 1];
+        const long long tn = an >= 0 ? (long long)(COHERENT ? load_acc(&A.sh_acc[an]) : A.sh_acc[an]) : 0;
+        const long long td = ad >= 0 ? (long long)(COHERENT ? load_acc(&A.sh_acc[ad]) : A.sh_acc[ad]) : 0;
+        const float num = (float)(A.sh_base[2 * k] + (double)tn * kFixInv);
+        const float den = (float)(A.sh_base[2 * k + 1] + (double)td * kFixInv);
+        A.sh_num_out[k] = num;
+        A.sh_den_out[k] = den;
+        A.sh_val_out[k] = clamp_value(num, den);
+    }
+    __syncthreads();
+    for (int j = tid; j < A.kacc; j += nthr) A.sh_acc[j] = 0ull;
+}
+
+// M-step of a per-qd entry whose sums several work items added to gacc; the
+// accumulators are cleared for the next iteration.
+template <int MODEL, bool COHERENT>
+__device__ __forceinline__ void finalize_merged_entry(const KArgs& A, int p) {
+    constexpr int T = Shape<MODEL>::T;
+    const int2 oc = A.counts[p];
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const float2 bd = base_den<MODEL>(t, oc);
+        const unsigned long long v = COHERENT ? load_acc(&A.tab[t].gacc[p]) : A.tab[t].gacc[p];
+        const float num = fix_to_float(bd.x, v);
+        A.tab[t].val_out[p] = clamp_value(num, bd.y);
+        if (A.write_num) A.tab[t].num_out[p] = num;
+        A.tab[t].gacc[p] = 0ull;
+    }
+}
+
 // Persistent form of the window path over items [first, first + n_win), all
 // packed (PACKED) or all rank-major: workgroup b of G resident ones
 // processes one window item per round, item kG + b in even rounds k and
@@ -380,6 +430,7 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
     extern __shared__ unsigned long long lds[];
     __shared__ float sh[K < 64 ? 64 : K];
     __shared__ long long red[KACC][4];
+    __shared__ int last_one;
 
     const int tid = threadIdx.x;
     const int Wmax = A.max_width;
@@ -493,6 +544,23 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
                 }
             }
         }
+        // Fused M-step of a split query group: the workgroup that adds the
+        // group's last slice finalises the group's entries.
+        if (A.fuse_merge && it.split >= 0) {
+            wait_memory();
+            __syncthreads();
+            const int4 g = A.split_groups[it.split];
+            if (tid == 0) {
+                last_one = atomicAdd(&A.split_done[it.split], 1u) == (unsigned)g.z - 1u;
+                wait_memory();
+            }
+            __syncthreads();
+            if (last_one) {
+                wait_memory();
+                for (int p = g.x + tid; p < g.y; p += NT) finalize_merged_entry<MODEL, true>(A, p);
+                if (tid == 0) A.split_done[it.split] = 0u;
+            }
+        }
         __syncthreads();
         round++;
         cur = nxt;
@@ -513,6 +581,21 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
             for (int c = 0; c < kUbmCopies; c++) v += (long long)slots[k * kUbmCopies + c];
         if (v != 0) atomicAdd(&A.sh_acc[k], (unsigned long long)v);
     }
+    // Fused M-step of the shared tables: the last workgroup to finish.
+    if (A.fuse_merge) {
+        wait_memory();
+        __syncthreads();
+        if (tid == 0) {
+            last_one = atomicAdd(A.grid_done, 1u) == (unsigned)G - 1u;
+            wait_memory();
+        }
+        __syncthreads();
+        if (last_one) {
+            wait_memory();
+            finalize_shared<true>(A, tid, NT);
+            if (tid == 0) *A.grid_done = 0u;
+        }
+    }
 }
 
 // Merge kernel: block 0 finalises the shared tables, the other blocks
@@ -523,19 +606,7 @@ __global__ void __launch_bounds__(256) em_merge_kernel(KArgs A) {
     constexpr int T = Shape<MODEL>::T;
     const int tid = threadIdx.x;
     if (blockIdx.x == 0) {
-        const int k = tid;
-        if (k < A.K) {
-            const int an = A.sh_acc_map[2 * k], ad = A.sh_acc_map[2 * k + 1];
-            const long long tn = an >= 0 ? (long long)A.sh_acc[an] : 0;
-            const long long td = ad >= 0 ? (long long)A.sh_acc[ad] : 0;
-            const float num = (float)(A.sh_base[2 * k] + (double)tn * kFixInv);
-            const float den = (float)(A.sh_base[2 * k + 1] + (double)td * kFixInv);
-            A.sh_num_out[k] = num;
-            A.sh_den_out[k] = den;
-            A.sh_val_out[k] = clamp_value(num, den);
-        }
-        __syncthreads();
-        for (int j = tid; j < A.kacc; j += blockDim.x) A.sh_acc[j] = 0ull;
+        finalize_shared<false>(A, tid, (int)blockDim.x);
         return;
     }
     const int64_t i = (int64_t)(blockIdx.x - 1) * blockDim.x + tid;
@@ -553,17 +624,7 @@ __global__ void __launch_bounds__(256) em_merge_kernel(KArgs A) {
         }
     } else {
         if (i >= A.n_shared) return;
-        const int p = A.shared_idx[i];
-        const int2 oc = A.counts[p];
-#pragma unroll
-        for (int t = 0; t < T; t++) {
-            const float2 bd = base_den<MODEL>(t, oc);
-            const unsigned long long v = A.tab[t].gacc[p];
-            const float num = fix_to_float(bd.x, v);
-            A.tab[t].val_out[p] = clamp_value(num, bd.y);
-            if (A.write_num) A.tab[t].num_out[p] = num;
-            A.tab[t].gacc[p] = 0ull;
-        }
+        finalize_merged_entry<MODEL, false>(A, A.shared_idx[i]);
     }
 }
 
@@ -671,6 +732,10 @@ KArgs make_args(mc_model* m, int cur, int write_num) {
     A.sh_val_out = m->d_sh_val[cur ^ 1];
     A.sh_num_out = m->d_sh_num[cur ^ 1];
     A.sh_den_out = m->d_sh_den[cur ^ 1];
+    A.fuse_merge = m->fuse ? 1 : 0;
+    A.split_groups = m->d_split_groups;
+    A.split_done = m->d_split_done;
+    A.grid_done = m->d_grid_done;
     return A;
 }
 
@@ -756,6 +821,7 @@ mc_status launch_estep(mc_model* m, int cur, int write_num) {
 }
 
 mc_status launch_mstep(mc_model* m, int cur, int write_num) {
+    if (m->fuse) return MC_OK;   // done by the persistent kernel
     KArgs A = make_args(m, cur, write_num);
     int64_t n_entries = m->reduce_mode == MC_REDUCE_ATOMIC ? m->n_qd : m->n_shared;
     unsigned grid = (unsigned)(1 + (n_entries + 255) / 256);
@@ -868,6 +934,9 @@ mc_status mc_model_set_option(mc_model* m, int option, int64_t value) {
         case MC_OPT_SKIP_CONSTANT:
             m->skip_constant = value != 0;
             break;
+        case MC_OPT_FUSED_MERGE:
+            m->fused_merge = value != 0;
+            break;
         case MC_OPT_BATCHES:
             MC_REQUIRE(value >= 1 && value <= 4096, "batches out of range [1, 4096]");
             m->n_batches = (int)value;
@@ -930,6 +999,12 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     // Persistent window kernel: 256-thread workgroups, windows staged through
     // registers (width <= 256 * kPersistPer), one workgroup per resident slot.
     m->persistent = m->persistent && m->block == 256 && max_width <= 256 * kPersistPer;
+    // The M-step runs inside the persistent kernel when that kernel is the
+    // iteration's only E-step launch (every work item a window item, all of
+    // one session format) in the segmented mode.
+    m->fuse = m->fused_merge && m->reduce_mode == MC_REDUCE_SEGMENTED && m->persistent && m->n_win_items > 0 &&
+              m->n_glob_items == 0 && m->n_dir_items == 0 &&
+              (m->n_pk_items == 0 || m->n_pk_items == m->n_win_items);
     if (m->persistent && m->n_win_items > 0) {
         int n_cu = 0;
         MC_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, m->device));
@@ -950,6 +1025,9 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     if (m->varlen) MC_TRY(dalloc(m, &m->d_len, h_len.size()));
     MC_TRY(dalloc(m, &m->d_items, items.size() * sizeof(Item)));
     MC_TRY(dalloc(m, &m->d_shared_idx, shared_idx.size() * 4));
+    MC_TRY(dalloc(m, &m->d_split_groups, std::max<size_t>(h.split_groups.size(), 1) * sizeof(int4)));
+    MC_TRY(dalloc(m, &m->d_split_done, std::max<size_t>(h.split_groups.size(), 1) * 4));
+    MC_TRY(dalloc(m, &m->d_grid_done, 4));
     MC_TRY(dalloc(m, &m->d_lists, lists.size() * 4));
     for (int t = 0; t < T; t++) {
         for (int b = 0; b < 2; b++) {
@@ -982,6 +1060,11 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     if (!shared_idx.empty())
         MC_HIP(hipMemcpyAsync(m->d_shared_idx, shared_idx.data(), shared_idx.size() * 4, hipMemcpyHostToDevice,
                               m->stream));
+    if (!h.split_groups.empty())
+        MC_HIP(hipMemcpyAsync(m->d_split_groups, h.split_groups.data(), h.split_groups.size() * sizeof(int4),
+                              hipMemcpyHostToDevice, m->stream));
+    MC_HIP(hipMemsetAsync(m->d_split_done, 0, std::max<size_t>(h.split_groups.size(), 1) * 4, m->stream));
+    MC_HIP(hipMemsetAsync(m->d_grid_done, 0, 4, m->stream));
     if (!lists.empty())
         MC_HIP(hipMemcpyAsync(m->d_lists, lists.data(), lists.size() * 4, hipMemcpyHostToDevice, m->stream));
     MC_HIP(hipMemcpyAsync(m->d_counts, counts.data(), counts.size() * 8, hipMemcpyHostToDevice, m->stream));

@@ -50,6 +50,8 @@ struct Item {
                         // sessions [e1, s1) contribute only constant counts (DBN and
                         // CCM sessions without a click) and are read by the gradient
                         // path alone
+    int32_t split;      // slices of one query group: the group's index in
+                        // KArgs::split_groups, else -1
 };
 
 struct TableArgs {
@@ -98,6 +100,13 @@ struct KArgs {
     float* sh_val_out;
     float* sh_num_out;
     float* sh_den_out;
+    // M-step inside the persistent kernel (fuse_merge): the last slice of a
+    // split query group to finish finalises the group's entries, the last
+    // workgroup the shared tables; counters return to 0 for the next launch.
+    int32_t fuse_merge;
+    const int4* split_groups;     // per split group: first entry, end entry, slices
+    unsigned* split_done;         // per split group: slices finished
+    unsigned* grid_done;          // workgroups finished
 };
 
 // Counts of an entry from its 2-byte packed form, or from the full array

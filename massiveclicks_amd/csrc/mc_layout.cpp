@@ -138,8 +138,10 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
         int32_t base = 0, width = 0, flags = 0;
         int kind = kKindWindow;
         bool packed = false;    // window item stored as 16-byte session records
+        int32_t split = -1;     // slice of split group `split` (window slices only)
         int64_t gstart = 0;     // first session (query order) of the item's first group
     };
+    std::vector<int4> split_groups;   // per window-sliced group: first entry, end entry, slices
     std::vector<Spec> specs;
     std::vector<int32_t> shared_idx, shared_batch;
     // Minibatches of the gradient path: contiguous runs of whole query
@@ -219,6 +221,8 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
                 } else {
                     close();
                     const int64_t n_slices = (gs + S - 1) / S;
+                    const int32_t split = gw <= W ? (int32_t)split_groups.size() : -1;
+                    if (gw <= W) split_groups.push_back(make_int4(g.lo, g.hi, (int)n_slices, 0));
                     for (int64_t k = 0; k < n_slices; k++) {
                         Spec sl;
                         const int64_t a = g.s0 + gs * k / n_slices, b = g.s0 + gs * (k + 1) / n_slices;
@@ -227,6 +231,7 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
                         sl.base = g.lo;
                         sl.width = gw <= W ? (int32_t)gw : 0;
                         sl.kind = gw <= W ? kKindWindow : kKindGlobal;
+                        sl.split = split;
                         sl.gstart = g.s0;
                         specs.push_back(sl);
                     }
@@ -245,6 +250,7 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
         m->reduce_mode = MC_REDUCE_ATOMIC;
         specs.clear();
         shared_idx.clear();
+        split_groups.clear();
         for (int64_t a = 0; a < n; a += S) {
             Spec c;
             c.ranges.push_back({a, std::min<int64_t>(n, a + S)});
@@ -319,7 +325,7 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
             while (e1 > j0 && clickless(order[(size_t)e1 - 1])) e1--;
         m->n_em_sessions += e1 - j0;
         items.push_back(Item{(int32_t)j0, (int32_t)order.size(), sp.base, sp.width, sp.flags, batch_of(sp.gstart),
-                             (int32_t)e1});
+                             (int32_t)e1, sp.split});
         if (sp.kind == kKindWindow && structured) {
             m->n_win_items++;
             if (sp.packed) m->n_pk_items++;
@@ -459,6 +465,7 @@ mc_status plan_layout(mc_model* m, int64_t n, const int32_t* query, const int32_
     h->items = std::move(items);
     h->shared_idx = std::move(shared_idx);
     h->lists = std::move(lists);
+    h->split_groups = std::move(split_groups);
     h->packed = std::move(h_packed);
     h->pidx16 = std::move(h_pidx16);
     h->pidx32 = std::move(h_pidx);

@@ -17,6 +17,7 @@ struct HostLayout {
     std::vector<Item> items;         // work items in launch order
     std::vector<int32_t> shared_idx; // entries merged across items, ordered by minibatch
     std::vector<int32_t> lists;      // per minibatch: its window items, then its global items
+    std::vector<int4> split_groups;  // per group sliced over window items: first entry, end entry, slices
     std::vector<uint32_t> packed;    // [n_pk][4] packed session records
     std::vector<uint16_t> pidx16;    // [10][n_w - n_pk] window-relative indices
     std::vector<int32_t> pidx32;     // [10][N - n_w] absolute indices

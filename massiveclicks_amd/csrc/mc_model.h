@@ -36,6 +36,8 @@ struct mc_model {
     int persistent = 1;
     int packed = 1;           // MC_OPT_PACKED
     int skip_constant = 1;    // MC_OPT_SKIP_CONSTANT
+    int fused_merge = 1;      // MC_OPT_FUSED_MERGE
+    bool fuse = false;        // the persistent kernel runs the M-step of this layout
     int persist_grid[2] = {0, 0};   // resident workgroups of the persistent window kernel: [0] rank-major, [1] packed items
     // shape
     int T = 1, K = 10, KACC = 10;
@@ -60,6 +62,9 @@ struct mc_model {
     int max_width = 0;
     int64_t max_item_sessions = 0;
     int32_t* d_shared_idx = nullptr;
+    int4* d_split_groups = nullptr;  // per split group: first entry, end entry, slices
+    unsigned* d_split_done = nullptr;
+    unsigned* d_grid_done = nullptr;
     int64_t n_shared = 0;
     float* d_val[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [buf][t]
     float* d_num[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};

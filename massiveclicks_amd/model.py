@@ -21,7 +21,7 @@ MODEL_NAMES = {PBM: "PBM", CCM: "CCM", DBN: "DBN", UBM: "UBM"}
 MAX_SERP = 10
 
 (OPT_REDUCE_MODE, OPT_ITEM_SESSIONS, OPT_ITEM_WINDOW, OPT_SORT_BY_LCR, OPT_BLOCK, OPT_DIRECT, OPT_PERSISTENT,
- OPT_BATCHES, OPT_PACKED, OPT_SKIP_CONSTANT) = range(10)
+ OPT_BATCHES, OPT_PACKED, OPT_SKIP_CONSTANT, OPT_FUSED_MERGE) = range(11)
 REDUCE_SEGMENTED, REDUCE_ATOMIC = 0, 1
 
 
@@ -130,7 +130,8 @@ class ClickModel:
     def __init__(self, model_type: int, device: int = 0, reduce_mode: int = REDUCE_SEGMENTED,
                  item_sessions: int | None = None, item_window: int | None = None,
                  sort_by_lcr: bool = True, block: int | None = None, direct: bool = False,
-                 persistent: bool = True, batches: int = 1, packed: bool = True, skip_constant: bool = True):
+                 persistent: bool = True, batches: int = 1, packed: bool = True, skip_constant: bool = True,
+                 fused_merge: bool = True):
         self.model_type = int(model_type)
         self._h = C.c_void_p()
         L = _lib.lib()
@@ -142,6 +143,7 @@ class ClickModel:
         check(L.mc_model_set_option(self._h, OPT_BATCHES, int(batches)))
         check(L.mc_model_set_option(self._h, OPT_PACKED, int(bool(packed))))
         check(L.mc_model_set_option(self._h, OPT_SKIP_CONSTANT, int(bool(skip_constant))))
+        check(L.mc_model_set_option(self._h, OPT_FUSED_MERGE, int(bool(fused_merge))))
         if item_sessions is not None:
             check(L.mc_model_set_option(self._h, OPT_ITEM_SESSIONS, int(item_sessions)))
         if item_window is not None:
@@ -368,9 +370,9 @@ class ClickModel:
 
 
 OPTION_NAMES = ("reduce_mode", "item_sessions", "item_window", "sort_by_lcr", "block", "direct", "persistent",
-                "batches", "packed", "skip_constant")
+                "batches", "packed", "skip_constant", "fused_merge")
 DEFAULT_OPTIONS = {"reduce_mode": REDUCE_SEGMENTED, "item_sessions": 8192, "item_window": 1280, "sort_by_lcr": 1,
-                   "block": 256, "direct": 0, "persistent": 1, "batches": 1, "packed": 1, "skip_constant": 1}
+                   "block": 256, "direct": 0, "persistent": 1, "batches": 1, "packed": 1, "skip_constant": 1, "fused_merge": 1}
 
 
 def autotune(model_type: int, train: SessionSet, n_qd: int, device: int = 0, iterations: int = 5,
@@ -405,6 +407,7 @@ def autotune(model_type: int, train: SessionSet, n_qd: int, device: int = 0, ite
     best["persistent"] = bool(best["persistent"])
     best["packed"] = bool(best["packed"])
     best["skip_constant"] = bool(best["skip_constant"])
+    best["fused_merge"] = bool(best["fused_merge"])
     if cache:
         known = {}
         if os.path.exists(cache):
