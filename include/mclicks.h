@@ -77,11 +77,12 @@ enum {
      * constant bases at init, so visiting them changes no result.  The
      * sessions stay on the device for the gradient path.  0 = visit them. */
     MC_OPT_SKIP_CONSTANT = 9,
-    /* 1 = when the persistent window kernel is an iteration's only E-step
-     * launch, it also runs the M-step: the workgroup that adds the last
-     * slice of a query group split over several work items finalises the
-     * group's entries, and the last workgroup to finish the shared tables,
-     * so no merge kernel is launched (default 1).  Same results. */
+    /* 1 = when the window kernel (persistent or one workgroup per item) is
+     * an iteration's only E-step launch, it also runs the M-step: the
+     * workgroup that adds the last slice of a query group split over several
+     * work items finalises the group's entries, and the last workgroup to
+     * finish the shared tables, so no merge kernel is launched (default 1).
+     * Same results. */
     MC_OPT_FUSED_MERGE = 10
 };
 enum { MC_REDUCE_SEGMENTED = 0, MC_REDUCE_ATOMIC = 1 };

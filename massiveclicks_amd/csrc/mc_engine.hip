@@ -186,6 +186,93 @@ __device__ __forceinline__ void run_item_direct(const KArgs& A, const Stream& st
     }
 }
 
+// Cross-workgroup hand-over inside one launch (fused M-step).  The data
+// handed over (fixed-point sums, counters) is only ever written with
+// device-scope atomics, which are performed at the device's coherence point;
+// so the writer only has to wait until its atomics have completed before it
+// bumps a counter (an agent-scope release fence would also write back the
+// XCD's L2 for plain stores, which nothing here needs and which cost 4x the
+// kernel's time), and the reader reads with device-coherent loads.
+__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+__device__ __forceinline__ unsigned long long load_acc(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// M-step of the shared (rank-indexed) tables from their fixed-point sums and
+// FP64 bases, threads tid < K; then the accumulators are cleared for the next
+// iteration.  COHERENT: the sums were added during this launch.
+template <bool COHERENT>
+__device__ __forceinline__ void finalize_shared(const KArgs& A, int tid, int nthr) {
+    const int k = tid;
+    if (k < A.K) {
+        const int an = A.sh_acc_map[2 * k], ad = A.sh_acc_map[2 * k + 1];
+        const long long tn = an >= 0 ? (long long)(COHERENT ? load_acc(&A.sh_acc[an]) : A.sh_acc[an]) : 0;
+        const long long td = ad >= 0 ? (long long)(COHERENT ? load_acc(&A.sh_acc[ad]) : A.sh_acc[ad]) : 0;
+        const float num = (float)(A.sh_base[2 * k] + (double)tn * kFixInv);
+        const float den = (float)(A.sh_base[2 * k + 1] + (double)td * kFixInv);
+        A.sh_num_out[k] = num;
+        A.sh_den_out[k] = den;
+        A.sh_val_out[k] = clamp_value(num, den);
+    }
+    __syncthreads();
+    for (int j = tid; j < A.kacc; j += nthr) A.sh_acc[j] = 0ull;
+}
+
+// M-step of a per-qd entry whose sums several work items added to gacc; the
+// accumulators are cleared for the next iteration.
+template <int MODEL, bool COHERENT>
+__device__ __forceinline__ void finalize_merged_entry(const KArgs& A, int p) {
+    constexpr int T = Shape<MODEL>::T;
+    const int2 oc = A.counts[p];
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const float2 bd = base_den<MODEL>(t, oc);
+        const unsigned long long v = COHERENT ? load_acc(&A.tab[t].gacc[p]) : A.tab[t].gacc[p];
+        const float num = fix_to_float(bd.x, v);
+        A.tab[t].val_out[p] = clamp_value(num, bd.y);
+        if (A.write_num) A.tab[t].num_out[p] = num;
+        A.tab[t].gacc[p] = 0ull;
+    }
+}
+
+// Fused M-step of a split query group after a workgroup has added one of
+// its slices: the workgroup that completes the group finalises its entries.
+// Block-uniform call (it.split >= 0); `flag` is a shared int.
+template <int MODEL>
+__device__ __forceinline__ void fused_split_merge(const KArgs& A, const Item& it, int tid, int nthr, int& flag) {
+    wait_memory();
+    __syncthreads();
+    const int4 g = A.split_groups[it.split];
+    if (tid == 0) {
+        flag = atomicAdd(&A.split_done[it.split], 1u) == (unsigned)g.z - 1u;
+        wait_memory();
+    }
+    __syncthreads();
+    if (flag) {
+        wait_memory();
+        for (int p = g.x + tid; p < g.y; p += nthr) finalize_merged_entry<MODEL, true>(A, p);
+        if (tid == 0) A.split_done[it.split] = 0u;
+    }
+}
+
+// Fused M-step of the shared tables: after a workgroup's shared-table sums
+// are in, the last of the launch's n_groups workgroups finalises the tables.
+__device__ __forceinline__ void fused_shared_merge(const KArgs& A, int tid, int nthr, unsigned n_groups, int& flag) {
+    wait_memory();
+    __syncthreads();
+    if (tid == 0) {
+        flag = atomicAdd(A.grid_done, 1u) == n_groups - 1u;
+        wait_memory();
+    }
+    __syncthreads();
+    if (flag) {
+        wait_memory();
+        finalize_shared<true>(A, tid, nthr);
+        if (tid == 0) *A.grid_done = 0u;
+    }
+}
+
 // Session paths, one kernel instantiation each (each gets its own register
 // allocation): window items in LDS (rank-major or packed sessions), items
 // without a window (global 64-bit fixed-point atomics), and the float-atomic
@@ -206,6 +293,7 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
     extern __shared__ unsigned long long lds[];
     __shared__ float sh[K < 64 ? 64 : K];
     __shared__ long long red[KACC][8];
+    __shared__ int last_one;
 
     const Item it = A.items[item_offset + blockIdx.x];
     const int tid = threadIdx.x;
@@ -335,55 +423,11 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
             }
         }
     }
-}
-
-// Cross-workgroup hand-over inside one launch (fused M-step).  The data
-// handed over (fixed-point sums, counters) is only ever written with
-// device-scope atomics, which are performed at the device's coherence point;
-// so the writer only has to wait until its atomics have completed before it
-// bumps a counter (an agent-scope release fence would also write back the
-// XCD's L2 for plain stores, which nothing here needs and which cost 4x the
-// kernel's time), and the reader reads with device-coherent loads.
-__device__ __forceinline__ void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
-
-__device__ __forceinline__ unsigned long long load_acc(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// M-step of the shared (rank-indexed) tables from their fixed-point sums and
-// FP64 bases, threads tid < K; then the accumulators are cleared for the next
-// iteration.  COHERENT: the sums were added during this launch.
-template <bool COHERENT>
-__device__ __forceinline__ void finalize_shared(const KArgs& A, int tid, int nthr) {
-    const int k = tid;
-    if (k < A.K) {
-        const int an = A.sh_acc_map[2 * k], ad = A.sh_acc_map[2 * k + 1];
-        const long long tn = an >= 0 ? (long long)(COHERENT ? load_acc(&A.sh_acc[an]) : A.sh_acc[an]) : 0;
-        const long long td = ad >= 0 ? (long long)(COHERENT ? load_acc(&A.sh_acc[ad]) : A.sh_acc[ad]) : 0;
-        const float num = (float)(A.sh_base[2 * k] + (double)tn * kFixInv);
-        const float den = (float)(A.sh_base[2 * k + 1] + (double)td * kFixInv);
-        A.sh_num_out[k] = num;
-        A.sh_den_out[k] = den;
-        A.sh_val_out[k] = clamp_value(num, den);
-    }
-    __syncthreads();
-    for (int j = tid; j < A.kacc; j += nthr) A.sh_acc[j] = 0ull;
-}
-
-// M-step of a per-qd entry whose sums several work items added to gacc; the
-// accumulators are cleared for the next iteration.
-template <int MODEL, bool COHERENT>
-__device__ __forceinline__ void finalize_merged_entry(const KArgs& A, int p) {
-    constexpr int T = Shape<MODEL>::T;
-    const int2 oc = A.counts[p];
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-        const float2 bd = base_den<MODEL>(t, oc);
-        const unsigned long long v = COHERENT ? load_acc(&A.tab[t].gacc[p]) : A.tab[t].gacc[p];
-        const float num = fix_to_float(bd.x, v);
-        A.tab[t].val_out[p] = clamp_value(num, bd.y);
-        if (A.write_num) A.tab[t].num_out[p] = num;
-        A.tab[t].gacc[p] = 0ull;
+    // Fused M-step (the launch is the iteration's only E-step launch): split
+    // groups, then the shared tables, as in the persistent kernel.
+    if (A.fuse_merge) {
+        if (it.split >= 0) fused_split_merge<MODEL>(A, it, tid, nthr, last_one);
+        fused_shared_merge(A, tid, nthr, gridDim.x, last_one);
     }
 }
 
@@ -546,21 +590,7 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
         }
         // Fused M-step of a split query group: the workgroup that adds the
         // group's last slice finalises the group's entries.
-        if (A.fuse_merge && it.split >= 0) {
-            wait_memory();
-            __syncthreads();
-            const int4 g = A.split_groups[it.split];
-            if (tid == 0) {
-                last_one = atomicAdd(&A.split_done[it.split], 1u) == (unsigned)g.z - 1u;
-                wait_memory();
-            }
-            __syncthreads();
-            if (last_one) {
-                wait_memory();
-                for (int p = g.x + tid; p < g.y; p += NT) finalize_merged_entry<MODEL, true>(A, p);
-                if (tid == 0) A.split_done[it.split] = 0u;
-            }
-        }
+        if (A.fuse_merge && it.split >= 0) fused_split_merge<MODEL>(A, it, tid, NT, last_one);
         __syncthreads();
         round++;
         cur = nxt;
@@ -582,20 +612,7 @@ em_window_persistent_kernel(KArgs A, int first, int n_win) {
         if (v != 0) atomicAdd(&A.sh_acc[k], (unsigned long long)v);
     }
     // Fused M-step of the shared tables: the last workgroup to finish.
-    if (A.fuse_merge) {
-        wait_memory();
-        __syncthreads();
-        if (tid == 0) {
-            last_one = atomicAdd(A.grid_done, 1u) == (unsigned)G - 1u;
-            wait_memory();
-        }
-        __syncthreads();
-        if (last_one) {
-            wait_memory();
-            finalize_shared<true>(A, tid, NT);
-            if (tid == 0) *A.grid_done = 0u;
-        }
-    }
+    if (A.fuse_merge) fused_shared_merge(A, tid, NT, (unsigned)G, last_one);
 }
 
 // Merge kernel: block 0 finalises the shared tables, the other blocks
@@ -999,10 +1016,11 @@ mc_status mc_init_parameters(mc_model* m, int64_t n, const int32_t* query, const
     // Persistent window kernel: 256-thread workgroups, windows staged through
     // registers (width <= 256 * kPersistPer), one workgroup per resident slot.
     m->persistent = m->persistent && m->block == 256 && max_width <= 256 * kPersistPer;
-    // The M-step runs inside the persistent kernel when that kernel is the
-    // iteration's only E-step launch (every work item a window item, all of
-    // one session format) in the segmented mode.
-    m->fuse = m->fused_merge && m->reduce_mode == MC_REDUCE_SEGMENTED && m->persistent && m->n_win_items > 0 &&
+    // The M-step runs inside the E-step kernel when that kernel is the
+    // iteration's only launch (every work item a window item, all of one
+    // session format) in the segmented mode: the persistent kernel or the
+    // one-workgroup-per-item kernel.
+    m->fuse = m->fused_merge && m->reduce_mode == MC_REDUCE_SEGMENTED && m->n_win_items > 0 &&
               m->n_glob_items == 0 && m->n_dir_items == 0 &&
               (m->n_pk_items == 0 || m->n_pk_items == m->n_win_items);
     if (m->persistent && m->n_win_items > 0) {

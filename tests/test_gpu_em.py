@@ -431,21 +431,23 @@ def test_skipping_constant_sessions_changes_nothing(model, uneven, persistent):
 
 @pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
 @pytest.mark.parametrize("uneven", [False, True], ids=["ten", "uneven"])
-def test_fused_merge_equals_merge_kernel(model, uneven):
-    """MC_OPT_FUSED_MERGE: the persistent kernel finalises the entries of
-    query groups split over several work items (256-session items split
-    every group of more than 256 sessions) and the shared tables itself,
-    instead of a merge kernel.  Same kernel, same sums: identical parameters
-    bit for bit, and within the bar of the oracle."""
+@pytest.mark.parametrize("shape", [dict(), dict(persistent=False, block=512)], ids=["persistent", "items-b512"])
+def test_fused_merge_equals_merge_kernel(model, uneven, shape):
+    """MC_OPT_FUSED_MERGE: the window kernel (persistent, or one workgroup
+    per item) finalises the entries of query groups split over several work
+    items (256-session items split every group of more than 256 sessions)
+    and the shared tables itself, instead of a merge kernel.  Same kernel,
+    same sums: identical parameters bit for bit, and within the bar of the
+    oracle."""
     if uneven:
         ds = varlen_workload(model)[0]
         train, n_qd, tabs = ds.train, ds.n_qd, varlen_workload(model)[1]
     else:
         d, tabs, _ = workload(model)
         train, n_qd = sset(d["train"]), d["n_qd"]
-    print(f"case fused-merge model={model} uneven={uneven}", flush=True)
-    with ClickModel(model, device=0, item_sessions=256) as a, \
-            ClickModel(model, device=0, item_sessions=256, fused_merge=False) as b:
+    print(f"case fused-merge model={model} uneven={uneven} {shape}", flush=True)
+    with ClickModel(model, device=0, item_sessions=256, **shape) as a, \
+            ClickModel(model, device=0, item_sessions=256, fused_merge=False, **shape) as b:
         for cm in (a, b):
             if uneven:
                 cm.init_from_dataset(ds)
