@@ -462,6 +462,61 @@ def test_fused_merge_equals_merge_kernel(model, uneven, shape):
             assert err < TOL_PARAM, f"table {t}: max |value - oracle| = {err}"
 
 
+@pytest.mark.parametrize("model", [PBM, UBM, DBN, CCM], ids=["PBM", "UBM", "DBN", "CCM"])
+@pytest.mark.parametrize("shape", [dict(), dict(persistent=False, block=512, item_sessions=256)],
+                         ids=["persistent", "items-b512"])
+def test_skipping_constant_sessions_across_parameter_changes(model, shape):
+    """MC_OPT_SKIP_CONSTANT across the API's state changes: with and without
+    skipping click-free sessions, every stage of a run that mixes graph
+    replays (fresh, cached and recaptured from each buffer parity), eager
+    launches, loaded and perturbed parameters, a reset and a step-wise
+    E/M iteration leaves identical parameters bit for bit."""
+    d = workload(model)[0]
+    train, n_qd = sset(d["train"]), d["n_qd"]
+    print(f"case skip-constant-sequence model={model} {shape}", flush=True)
+    with ClickModel(model, device=0, **shape) as a, \
+            ClickModel(model, device=0, skip_constant=False, **shape) as b:
+        pair = (a, b)
+
+        def same(stage):
+            for t in range(a.n_tables()):
+                np.testing.assert_array_equal(a.get_values(t), b.get_values(t), err_msg=f"{stage}, table {t}")
+                np.testing.assert_array_equal(a.get_parameters(t)[0], b.get_parameters(t)[0],
+                                              err_msg=f"{stage}, table {t}")
+
+        for cm in pair:
+            cm.init_parameters(train, n_qd)
+            cm.em(3)
+        same("graph from fresh buffers")
+        for cm in pair:
+            cm.em(4, use_graph=False)
+        same("eager")
+        ckpt = [a.get_parameters(t) for t in range(a.n_tables())]
+        for cm in pair:
+            cm.em(2)
+        same("graph from written buffers")
+        rng = np.random.default_rng(5)
+        for t, (num, den) in enumerate(ckpt):
+            num = (num * rng.uniform(0.5, 1.0, size=num.shape)).astype(np.float32)
+            for cm in pair:
+                cm.set_parameters(t, num, den)
+        for cm in pair:
+            cm.em(5)
+        same("after set_parameters")
+        for cm in pair:
+            cm.reset_parameters()
+            cm.em(6, use_graph=False)
+        same("after reset_parameters")
+        for cm in pair:
+            cm.process_session()
+            cm.update_parameters()
+            cm.em(5)
+        same("after a step-wise iteration")
+        for cm in pair:
+            cm.em(5)
+        same("cached graph")
+
+
 VARLEN_LAYOUTS = {
     "items-b256": dict(persistent=False, block=256),
     "items-b512": dict(persistent=False, block=512),
