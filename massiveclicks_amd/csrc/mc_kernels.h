@@ -148,10 +148,15 @@ __device__ __forceinline__ long long to_fix(float x) {
 
 // to_fix for a posterior x in [0, 1]: every float below 1 is at most
 // 1 - 2^-24, so x * 2^32 <= 2^32 - 256 converts exactly and the result equals
-// to_fix(x); the bound only changes x = 1 (by 256 * 2^-32).  Three
-// instructions instead of five.
+// to_fix(x).  v_cvt_u32_f32 saturates inputs above the u32 range to
+// 2^32 - 1 (and NaN to 0), so x >= 1, which the hardware reciprocal can give
+// for a posterior of exactly 1, becomes 1 - 2^-32 with no separate clamp;
+// written as asm because a float-to-unsigned conversion out of range is
+// undefined in C++.  Two instructions instead of five.
 __device__ __forceinline__ unsigned long long to_fix_unit(float x) {
-    return (unsigned long long)(uint32_t)fminf(x * kFixScale, 4294967040.0f);
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x * kFixScale));
+    return (unsigned long long)r;
 }
 
 // a / b through the hardware reciprocal (1 ulp); used in the DBN and CCM
@@ -364,16 +369,18 @@ __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t
         }
         return;
     }
-    int slot[kL];
-    ubm_slots(mm, slot);
-    const int fc = __builtin_ctz(mm);   // first click; ranks before it have no earlier click
+    // The previous click of rank r follows the ranks in order (kL - 1 until
+    // the first click), so each rank's slot costs one select.
+    int prev = kL - 1;
 #pragma unroll
     for (int r = 0; r < kL; r++) {
-        if (r < L && !((mm >> r) & 1u)) {
-            const int row = r * (r + 1) / 2;
+        const bool clicked = (mm >> r) & 1u;
+        const int row = r * (r + 1) / 2;
+        const int slot = row + prev;
+        if (r < L && !clicked) {
             const int p = pv[r];
             float atr = src.get(0, p);
-            float ex = sh[slot[r]];
+            float ex = sh[slot];
             float ae = atr * ex;
             const float inv = __builtin_amdgcn_rcpf(1 - ae);
             float na = (atr - ae) * inv;
@@ -383,14 +390,15 @@ __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t
                 // Before the first click the slot is the row's "no earlier
                 // click" one, known at compile time (a register, as for
                 // click-free sessions); after it, one LDS add into the slot.
-                if (r < fc) acc[row + kL - 1] += ne;
-                else em.add_slot(slot[r], ne);
+                if (prev == kL - 1) acc[row + kL - 1] += ne;
+                else em.add_slot(slot, ne);
             } else {
 #pragma unroll
-                for (int j = 0; j < r; j++) acc[row + j] += (slot[r] == row + j) ? ne : 0.f;
-                acc[row + kL - 1] += (slot[r] == row + kL - 1) ? ne : 0.f;
+                for (int j = 0; j < r; j++) acc[row + j] += (slot == row + j) ? ne : 0.f;
+                acc[row + kL - 1] += (slot == row + kL - 1) ? ne : 0.f;
             }
         }
+        prev = clicked ? r : prev;
     }
 }
 
