@@ -52,16 +52,11 @@ __device__ __forceinline__ void run_session_len(const int (&pv)[kL], uint32_t m,
         ubm_session(pv, LC, mm, src, em, sh, acc);
     } else if (MODEL == kDBN) {
         if (mm == 0) return;   // every count of a click-free session is a constant
-        float gn = 0.f, gd = 0.f;
-        dbn_session<VARLEN>(pv, LC, mm, src, em, sh[0], gn, gd);
-        acc[0] += gn;
-        acc[1] += gd;
+        // The gamma and tau sums go straight into the lane's accumulators.
+        dbn_session<VARLEN>(pv, LC, mm, src, em, sh[0], acc[0], acc[1]);
     } else {
         if (mm == 0) return;
-        float ta[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        ccm_session<VARLEN>(pv, LC, mm, src, em, sh[0], sh[1], sh[2], ta);
-#pragma unroll
-        for (int k = 0; k < 6; k++) acc[k] += ta[k];
+        ccm_session<VARLEN>(pv, LC, mm, src, em, sh[0], sh[1], sh[2], *reinterpret_cast<float(*)[6]>(&acc[0]));
     }
 }
 

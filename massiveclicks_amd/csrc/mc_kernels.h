@@ -489,7 +489,8 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
             else Tn = (r + 1 < kL) ? V[(kL - 1 - r) < 0 ? 0 : (kL - 1 - r)] : 1.f;
             if (r == lcr) {
                 const float sv = src.get(1, pv[r]);
-                em.add_wide(1, r, pv[r], fdiv(sv, 1 - (1 - sv) * g * car));
+                // At most 1 (g car <= 1), so the unit conversion applies.
+                em.add(1, r, pv[r], fdiv(sv, 1 - (1 - sv) * g * car));
                 const float phi4 = (1 - sv) * (1 - g);
                 const float phi5 = (1 - sv) * g * Tn;
                 const float inv = __builtin_amdgcn_rcpf(phi4 + phi5 + sv);
