@@ -309,6 +309,8 @@ struct SrcGlobal {          // straight from HBM, absolute indices
 template <class Src, class Emit>
 __device__ __forceinline__ void pbm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             const float* sh, float (&acc)[55]) {
+    // ae = a e enters 1 - ae, a - ae and e - ae as multiply-adds.
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int r = 0; r < kL; r++) {
         if (r < L && !((mm >> r) & 1u)) {
@@ -349,6 +351,8 @@ __device__ __forceinline__ void ubm_slots(uint32_t mm, int (&slot)[kL]) {
 template <class Src, class Emit>
 __device__ __forceinline__ void ubm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
                                             const float* sh, float (&acc)[55]) {
+    // ae = a e enters 1 - ae, a - ae and e - ae as multiply-adds.
+#pragma clang fp contract(fast)
     if (mm == 0) {
         // No click: every rank's slot is its row's "no previous click" one,
         // known at compile time, so the select over the row is not needed.
