@@ -114,7 +114,8 @@ __device__ __forceinline__ void decode_packed_test(uint4 v, int& L, int (&p)[kL]
 enum { kEvalSessionMajor = 0, kEvalRankMajor = 1, kEvalPacked = 2 };
 
 // The per-session arithmetic, the reference's operation for operation in
-// FP32: log conditional click probabilities lp, full click probabilities
+// FP32: log conditional click probabilities lp (log2 instead of the natural
+// log when !NATLOG, see log_click), full click probabilities
 // fp and the SERP click probabilities sp of the first L ranks; click bits cm.
 // The session's parameter values: attractiveness a and (DBN) satisfaction
 // sv of its first L ranks.
@@ -128,7 +129,14 @@ __device__ __forceinline__ void gather_values(const EvalArgs& E, int L, const in
     }
 }
 
-template <int MODEL>
+// Log of a conditional click probability: the natural log (the reference's
+// value, for per-session outputs) or, when only the sums are wanted, log2,
+// which costs half the instructions; the caller scales a session's log2 sum
+// by ln 2 once.
+template <bool NATLOG>
+__device__ __forceinline__ float log_click(float x) { return NATLOG ? logf(x) : log2f(x); }
+
+template <int MODEL, bool NATLOG>
 __device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const float (&a)[kL], const float (&sv)[kL],
                                              uint32_t cm, float (&lp)[kL], float (&fp)[kL], float (&sp)[kL]) {
     bool c[kL];
@@ -139,7 +147,7 @@ __device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const flo
         for (int r = 0; r < kL; r++) {
             if (r < L) {
                 float ae = a[r] * E.sh[r];
-                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
+                lp[r] = c[r] ? log_click<NATLOG>(ae) : log_click<NATLOG>(1 - ae);
                 fp[r] = c[r] ? ae : 1 - ae;
                 sp[r] = ae;
             }
@@ -156,7 +164,7 @@ __device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const flo
         for (int r = 0; r < kL; r++) {
             if (r < L) {
                 float ae = a[r] * E.sh[r * (r + 1) / 2 + prev[r]];
-                lp[r] = c[r] ? logf(ae) : logf(1 - ae);
+                lp[r] = c[r] ? log_click<NATLOG>(ae) : log_click<NATLOG>(1 - ae);
                 sp[r] = ae;
             }
         }
@@ -199,7 +207,7 @@ __device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const flo
                 float cp;
                 if (c[r]) { cp = a[r] * ex; ex = g * (1 - sv[r]); }
                 else { cp = 1 - a[r] * ex; ex *= g * (1 - a[r]) / cp; }
-                lp[r] = logf(cp);
+                lp[r] = log_click<NATLOG>(cp);
                 sp[r] = cp;
             }
         }
@@ -221,7 +229,7 @@ __device__ __forceinline__ void eval_session(const EvalArgs& E, int L, const flo
                 float cp;
                 if (c[r]) { cp = a[r] * ex; ex = t2 * (1 - a[r]) + t3 * a[r]; }
                 else { cp = 1 - a[r] * ex; ex *= t1 * (1 - a[r]) / cp; }
-                lp[r] = logf(cp);
+                lp[r] = log_click<NATLOG>(cp);
                 sp[r] = cp;
             }
         }
@@ -252,9 +260,12 @@ template <int MODEL, int LAYOUT, bool OUT>
 __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
     __shared__ double red[21][4];
     const int tid = threadIdx.x;
-    double v[11];
+    // Per-lane FP32 partials of the lane's sessions (a lane takes one
+    // session in every n / (blocks x 256), about four for 2M test sessions),
+    // FP64 from the wavefront sums on.
+    float v[11];
 #pragma unroll
-    for (int k = 0; k < 11; k++) v[k] = 0.0;
+    for (int k = 0; k < 11; k++) v[k] = 0.f;
     int cnt[kL];
 #pragma unroll
     for (int r = 0; r < kL; r++) cnt[r] = 0;
@@ -296,19 +307,20 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
         take(L, c, a, sv);
         if (i + stride < E.n) load(i + stride);
         float lp[kL], fp[kL], sp[kL];
-        eval_session<MODEL>(E, L, a, sv, c, lp, fp, sp);
+        eval_session<MODEL, OUT>(E, L, a, sv, c, lp, fp, sp);
         float acc = 0.f;
 #pragma unroll
         for (int r = 0; r < kL; r++)
             if (r < L) acc = acc + lp[r];
+        if (!OUT) acc *= 0.693147180559945309f;   // log2 sum to natural log
         const float s = acc / L;                 // evaluation.cpp:36
-        v[0] += (double)s;
+        v[0] += s;
         const int64_t o = (OUT && LAYOUT != kEvalSessionMajor && E.perm) ? (int64_t)E.perm[i] : i;
         if (OUT && E.llh) E.llh[o] = s;
 #pragma unroll
         for (int r = 0; r < kL; r++) {
             cnt[r] += (int)__popcll(__ballot(r < L));
-            if (r < L) v[1 + r] += (double)log2f(fp[r]);   // evaluation.cpp:62
+            if (r < L) v[1 + r] += log2f(fp[r]);   // evaluation.cpp:62
             if (OUT && E.logp) E.logp[o * kL + r] = r < L ? lp[r] : 0.f;
             if (OUT && E.full) E.full[o * kL + r] = r < L ? fp[r] : 0.f;
             if (OUT && E.prob) E.prob[o * kL + r] = r < L ? sp[r] : 0.f;
@@ -316,7 +328,7 @@ __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
     }
 #pragma unroll
     for (int k = 0; k < 11; k++) {
-        double w = wave_sum_d(v[k]);
+        double w = wave_sum_d((double)v[k]);
         if ((tid & 63) == 0) red[k][tid >> 6] = w;
     }
     if ((tid & 63) == 0)
