@@ -359,12 +359,17 @@ __global__ void __launch_bounds__(256) eval_reduce_kernel(const double* partial,
 }
 
 // Evaluation blocks per launch: at most this many (grid-stride beyond; each
-// lane then has its next session's loads in flight).  Measured on 1.94M
-// held-out sessions: 2048 blocks 49.8 us (DBN), against 53 us with 4096 and
-// 50 us with 1024.
+// lane then has its next session's loads in flight; 4096 blocks measured
+// slower).
 constexpr int64_t kEvalMaxBlocks = 2048;
 
-int64_t eval_blocks(int64_t n) { return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, kEvalMaxBlocks)); }
+// Blocks for n sessions of a model: the UBM kernel measured fastest at 1536
+// (46.1 us against 49.1 us with 2048 on the 1.94M-session set), the others
+// at 2048 (DBN 45.4 us; 45.9 with 1792, 46.7 with 1536, 48.0 with 1024).
+int64_t eval_blocks(int64_t n, int model_type) {
+    const int64_t cap = model_type == MC_UBM ? 1536 : kEvalMaxBlocks;
+    return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap));
+}
 
 // Scratch of mc_evaluate_device, the entry point for callers that cannot
 // allocate device memory: the per-block FP64 partials of one evaluation
@@ -454,7 +459,7 @@ void unpack_totals(const double (&out)[21], double* llh_sum, double* ppl_log2_su
 mc_status evaluate_host_arrays(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
                                float* logp, float* full, float* llh, float* prob, double (&out)[21]) {
     MC_HIP(hipSetDevice(m->device));
-    const int64_t nb = eval_blocks(n);
+    const int64_t nb = eval_blocks(n, m->type);
     std::vector<void*> tmp;
     auto talloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
@@ -544,7 +549,7 @@ mc_status mc_evaluate_device(mc_model* m, int64_t n, const int32_t* pidx, const 
     MC_REQUIRE(pidx && clicks && totals, "null argument");
     MC_REQUIRE(n >= 1 && n <= kEvalDeviceMaxSessions,
                "n out of range [1, " + std::to_string(kEvalDeviceMaxSessions) + "] for mc_evaluate_device");
-    const int64_t nb = eval_blocks(n);
+    const int64_t nb = eval_blocks(n, m->type);
     MC_REQUIRE((size_t)nb * 21 <= kScratchDoubles, "internal: evaluation partials exceed the scratch array");
     double* part = eval_scratch(m->device);
     MC_REQUIRE(part, "internal: evaluation scratch not set up on this device");
@@ -647,7 +652,7 @@ mc_status mc_set_test_set(mc_model* m, int64_t n, const int32_t* pidx, const uin
     m->d_test_part = nullptr;
     m->n_test = 0;
     const size_t before = m->mem_bytes;
-    const int64_t nb = eval_blocks(n);
+    const int64_t nb = eval_blocks(n, m->type);
     if (packable) {
         MC_TRY(dalloc(m, &m->d_test_packed, h_pk.size() * 4));
     } else {
@@ -677,7 +682,7 @@ mc_status mc_evaluate_test_set(mc_model* m, float* logp, float* full, float* llh
     MC_REQUIRE(m->n_test > 0, "no test set: call mc_set_test_set first");
     MC_HIP(hipSetDevice(m->device));
     const int64_t n = m->n_test;
-    const int64_t nb = eval_blocks(n);
+    const int64_t nb = eval_blocks(n, m->type);
     std::vector<void*> tmp;
     auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
     auto talloc = [&](size_t bytes) -> float* {
