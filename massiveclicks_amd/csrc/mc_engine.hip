@@ -387,15 +387,11 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
         }
     }
     // Flush the LDS window, four entries per thread per pass with the
-    // constant loads issued together: the packed counts, then the full
-    // counts of entries with 255 or more occurrences without a branch (other
-    // lanes read the window's first entry), so that no store waits for a
-    // count load issued after an earlier store.
+    // constant loads issued together.
     if (win && !fwin) {
         const bool interior = (it.flags & kItemInterior) != 0;
         for (int i0 = 0; i0 < W; i0 += 4 * nthr) {
             uint32_t pk[4];
-            int2 full[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int i = i0 + u * nthr + tid;
@@ -404,14 +400,9 @@ __global__ void __launch_bounds__(512) em_item_kernel(KArgs A, int item_offset) 
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int i = i0 + u * nthr + tid;
-                full[u] = A.counts[pk[u] == kCountsEscape ? it.base + i : it.base];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int i = i0 + u * nthr + tid;
                 if (i >= W) continue;
                 const int p = it.base + i;
-                const int2 oc = interior ? unpack_counts(pk[u], full[u]) : make_int2(0, 0);
+                const int2 oc = interior ? unpack_counts(pk[u], A.counts, p) : make_int2(0, 0);
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     const unsigned long long v = aw.sum(t, i);

@@ -479,22 +479,12 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
     if (WIN) {
         AdamStep ad;
         if (ADAM && interior) ad = adam_step(G, G.steps[0] + 1);
-        // Full counts of entries with 255 or more occurrences, loaded before
-        // the first store and without a branch (other lanes read the
-        // window's first entry): a conditional load between the stores made
-        // each entry wait for the previous entry's stores.
-        int2 f_full[PERA];
-#pragma unroll
-        for (int u = 0; u < PER; u++) {
-            const int i = u * NT + tid;
-            f_full[u] = G.counts[f_oc[u] == kCountsEscape ? it.base + i : it.base];
-        }
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int i = u * NT + tid;
             if (!interior || i >= W) continue;
             const int p = it.base + i;
-            const int2 oc = unpack_counts(f_oc[u], f_full[u]);
+            const int2 oc = unpack_counts(f_oc[u], G.counts, p);
 #pragma unroll
             for (int t = 0; t < T; t++) {
                 const float g = entry_gradient<MODEL>(t, fix_to_double(lacc[i * T + t]), oc, lval[i * T + t],
@@ -528,15 +518,12 @@ __global__ void __launch_bounds__(NT) grad_item_kernel(GArgs G, int64_t list_off
                     av[u][t] = (ADAM && on) ? G.av[t][p] : 0.f;
                 }
             }
-            int2 full[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) full[u] = G.counts[oc[u] == kCountsEscape ? it.base + i0 + u * NT : it.base];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int i = i0 + u * NT;
                 if (i >= W) continue;
                 const int p = it.base + i;
-                const int2 ocu = interior ? unpack_counts(oc[u], full[u]) : make_int2(0, 0);
+                const int2 ocu = interior ? unpack_counts(oc[u], G.counts, p) : make_int2(0, 0);
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                     const unsigned long long v = lacc[i * T + t];

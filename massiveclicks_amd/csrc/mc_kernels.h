@@ -109,10 +109,14 @@ struct KArgs {
     unsigned* grid_done;          // workgroups finished
 };
 
-// Counts of an entry from its 2-byte packed form, or, when they do not fit
-// (o >= 255; clicked occurrences never exceed o), its full counts `full`
-// (the caller loads them for every entry, without a branch).
+// Counts of an entry from its 2-byte packed form, or from the full array
+// when they do not fit (o >= 255; clicked occurrences never exceed o).
 constexpr uint32_t kCountsEscape = 0xFFFFu;
+__device__ __forceinline__ int2 unpack_counts(uint32_t pk, const int2* counts, int p) {
+    return pk != kCountsEscape ? make_int2((int)(pk & 0xFFu), (int)(pk >> 8)) : counts[p];
+}
+// The same with the full counts `full` already loaded (the persistent
+// kernel's flush loads them for every entry slot, without a branch).
 __device__ __forceinline__ int2 unpack_counts(uint32_t pk, int2 full) {
     return pk != kCountsEscape ? make_int2((int)(pk & 0xFFu), (int)(pk >> 8)) : full;
 }
