@@ -56,7 +56,7 @@ __device__ __forceinline__ void run_session_len(const int (&pv)[kL], uint32_t m,
         dbn_session<VARLEN>(pv, LC, mm, src, em, sh[0], acc[0], acc[1]);
     } else {
         if (mm == 0) return;
-        ccm_session<VARLEN>(pv, LC, mm, src, em, sh[0], sh[1], sh[2], *reinterpret_cast<float(*)[6]>(&acc[0]));
+        ccm_session<VARLEN>(pv, LC, mm, src, em, sh[0], sh[1], sh[2], acc);
     }
 }
 

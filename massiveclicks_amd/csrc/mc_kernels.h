@@ -538,7 +538,7 @@ __device__ __forceinline__ void dbn_session(const int (&pv)[kL], int L, uint32_t
 //   same tolerance.
 template <bool VARLEN, class Src, class Emit>
 __device__ __forceinline__ void ccm_session(const int (&pv)[kL], int L, uint32_t mm, const Src& src, Emit& em,
-                                            float t1, float t2, float t3, float (&ta)[6]) {
+                                            float t1, float t2, float t3, float* ta) {
 #pragma clang fp contract(fast)
     const int lcr = last_click(mm);
     // Pass 1, ranks 0..L-1: exam[r] (compute_exm_car) and the tail products
