@@ -199,9 +199,12 @@ mc_status mc_reset_parameters(mc_model* m);
  * Perplexity (click_models/evaluation.cpp:25-66).  pidx[n*10] (-1 = unseen
  * pair, valued 1/2), clicks[n*10], len[n] (NULL = 10).  Outputs: logp/full
  * (n*10, nullable), llh[n] per-session mean log-probability (nullable),
- * *llh_sum = sum of llh in FP64, ppl_log2_sum[10] = per-rank sums of
- * log2(full) in FP64, rank_count[10] = sessions contributing to each rank.
- * LL = llh_sum / n; PPL_r = 2^(-ppl_log2_sum[r] / rank_count[r]). */
+ * *llh_sum = sum of llh, ppl_log2_sum[10] = per-rank sums of log2(full)
+ * (FP32 partials per device lane over its few sessions, FP64 beyond),
+ * rank_count[10] = sessions contributing to each rank.  Without per-session
+ * outputs a session's llh is formed from log2 and scaled by ln 2 (within
+ * 1e-8 relative of the natural-log sum).  LL = llh_sum / n;
+ * PPL_r = 2^(-ppl_log2_sum[r] / rank_count[r]). */
 mc_status mc_evaluate(mc_model* m, int64_t n, const int32_t* pidx, const uint8_t* clicks, const uint8_t* len,
                       float* logp, float* full, float* llh, double* llh_sum, double* ppl_log2_sum,
                       int64_t* rank_count);
