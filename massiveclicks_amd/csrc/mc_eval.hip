@@ -260,9 +260,9 @@ template <int MODEL, int LAYOUT, bool OUT>
 __global__ void __launch_bounds__(256) eval_kernel(EvalArgs E) {
     __shared__ double red[21][4];
     const int tid = threadIdx.x;
-    // Per-lane FP32 partials of the lane's sessions (a lane takes one
-    // session in every n / (blocks x 256), about four for 2M test sessions),
-    // FP64 from the wavefront sums on.
+    // Per-lane FP32 partials of the lane's sessions (a lane takes every
+    // (blocks x 256)-th session, about four of 2M test sessions), FP64 from
+    // the wavefront sums on.
     float v[11];
 #pragma unroll
     for (int k = 0; k < 11; k++) v[k] = 0.f;
