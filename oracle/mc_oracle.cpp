@@ -18,10 +18,11 @@
 //   * PBM/UBM E-step ................. brute-force posterior enumeration
 //   * UBM slot index (Q5) ............ explicit table + brute force over 1024 masks
 //   * DBN/CCM exam/car, atr/sat ...... brute-force enumeration of hidden states
-//   * DBN/CCM tail chain + factors ... brute force for the start-0 chain (textbook
-//                                      part); the misaligned start>0 chains (Q2) and
-//                                      the factor masking are "parity unpinned"
-//                                      beyond the line-by-line restatement
+//   * DBN/CCM tail chain + factors ... brute-force enumeration: the posterior when
+//                                      all ranks share parameters, the start>0
+//                                      chains scored as a fresh list (Q2), and
+//                                      the masking of a click-free session
+//                                      (tests/test_oracle_pins.py)
 //   * M-step ......................... invariant theta = (1,2) + sum(tmp)
 //   * parser/split/sort/qd-index ..... hand-written raw-text fixtures (tests/golden)
 //   * evaluation ..................... closed forms + brute-force full click probs
