@@ -365,7 +365,8 @@ constexpr int64_t kEvalMaxBlocks = 2048;
 
 // Blocks for n sessions of a model: the UBM kernel measured fastest at 1536
 // (46.1 us against 49.1 us with 2048 on the 1.94M-session set), the others
-// at 2048 (DBN 45.4 us; 45.9 with 1792, 46.7 with 1536, 48.0 with 1024).
+// at or near 2048 (DBN 45.4 us; 45.9 with 1792, 46.7 with 1536, 48.0 with
+// 1024; CCM 41.3 us, 42.7 with 1536; PBM 30.6 us, 30.1 with 1792).
 int64_t eval_blocks(int64_t n, int model_type) {
     const int64_t cap = model_type == MC_UBM ? 1536 : kEvalMaxBlocks;
     return std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap));
